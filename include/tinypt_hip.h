@@ -1,0 +1,225 @@
+/*
+ * tinypt_hip.h — C ABI of the MI355X (gfx950) path-tracing backend, `device=hip`.
+ *
+ * This is the drop-in boundary that sits beside tinypt's `cpu` and `cuda`
+ * backends.  Every entry point takes plain pointers and sizes; no C++ or torch
+ * type crosses it.  The scene is handed over as a flat description
+ * (`tpt_scene_desc`) built on the host from tinypt's object model; the library
+ * builds its own acceleration structure, uploads one device arena and renders.
+ *
+ * Reference interfaces replaced (li-plus/tinypt):
+ *   tpt_hip_scene_create / _destroy  <- tinypt::Scene::to(Device)           src/tinypt/scene.h:24-37
+ *                                      cuda::Scene(const cpu::Scene&)       src/tinypt/cuda/scene.h:95-197
+ *   tpt_hip_render                   <- tinypt::PathTracer::render()        src/tinypt/path_tracer.h:26-44
+ *                                      cuda::PathTracer::render             src/tinypt/cuda/path_tracer.cpp:8-11
+ *   tpt_hip_trace_closest            <- cpu::Scene::intersect (closest hit) src/tinypt/cpu/scene.h:27
+ *   tpt_hip_trace_occluded           <- cpu::Scene::intersect_t (any hit)   src/tinypt/cpu/scene.h:28
+ *   tpt_hip_last_error               <- CHECK_CUDA -> std::runtime_error    src/tinypt/cuda/defs.h:33-40
+ *
+ * Conventions shared with the reference CPU path (src/tinypt/cpu/):
+ *   - z-up world, rotations stored as 3x3 with COLUMNS u, v, w (cpu/mapping.h:14-41);
+ *   - images are row-major with row 0 at the BOTTOM (y up), as cpu::Image (cpu/image.h:13-94);
+ *   - INF = 1e20, EPS = 1e-3 (cpu/defs.h:23-24).
+ *
+ * Thread-safety: one render per scene handle at a time; several handles may share a device.
+ */
+#ifndef TINYPT_HIP_H_
+#define TINYPT_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPT_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+enum {
+    TPT_OK = 0,
+    TPT_ERR_INVALID_ARGUMENT = -1, /* maps to std::invalid_argument */
+    TPT_ERR_RUNTIME = -2,          /* maps to std::runtime_error    */
+    TPT_ERR_HIP = -3,              /* HIP runtime failure (no device, OOM, launch) */
+    TPT_ERR_UNSUPPORTED = -4       /* object / material / light kind not supported */
+};
+
+/* ---- textures ------------------------------------------------------------ */
+enum { TPT_IMG_F32 = 0, TPT_IMG_U8 = 1 };
+
+/* One texel image.  `channels` is 1 (alpha / height), 2 (bump gradient) or 3/4
+ * (rgb / rgba).  U8 texels are read as value/255.  Row 0 is the bottom row. */
+typedef struct {
+    int32_t width, height, channels, format;
+    const void *data;
+} tpt_image;
+
+/* RGBTexture / AlphaTexture (cpu/texture.h:12-63): a constant or a map.
+ * For an alpha texture read from an RGBA image, `channel` selects the component. */
+typedef struct {
+    int32_t image;  /* -1: constant `value`; else index into tpt_scene_desc.images */
+    int32_t channel;/* first channel read from the image (0 for rgb, 3 for rgba alpha) */
+    float value[3]; /* rgb constant; an alpha constant uses value[0] */
+} tpt_texture;
+
+/* ---- materials (cpu/material.h:21-216) ------------------------------------ */
+enum {
+    TPT_BXDF_LAMBERTIAN = 0,
+    TPT_BXDF_METAL = 1,
+    TPT_BXDF_DIELECTRIC = 2,
+    TPT_BXDF_GLOSSY_BLINN = 3, /* Glossy + BlinnPhongModel(shininess)        */
+    TPT_BXDF_GLOSSY_GGX = 4    /* Glossy + TrowbridgeReitzModel(2/(s+2))     */
+};
+typedef struct {
+    int32_t type;
+    float param; /* dielectric: ior; glossy: shininess */
+    tpt_texture color;
+} tpt_bxdf;
+
+enum { TPT_COMBINE_NONE = 0, TPT_COMBINE_ADD = 1, TPT_COMBINE_MIX = 2 };
+typedef struct {
+    tpt_bxdf lobe[2];     /* lobe[1] used only when combine != NONE (AddBXDF / MixBXDF) */
+    int32_t combine;
+    float weight;         /* MixBXDF weight of lobe[0] */
+    tpt_texture emission; /* RGBTexture */
+    tpt_texture alpha;    /* AlphaTexture; constant 1 = opaque */
+    int32_t bump_image;   /* -1, or index of a 2-channel F32 gradient image (Sobel of the height map) */
+} tpt_material;
+
+/* ---- objects (cpu/object.h:22-291) ---------------------------------------- */
+enum { TPT_OBJ_SPHERE = 0, TPT_OBJ_RECTANGLE = 1, TPT_OBJ_CIRCLE = 2, TPT_OBJ_MESH = 3 };
+typedef struct {
+    int32_t type;
+    int32_t material; /* analytic objects: index into materials */
+    int32_t mesh;     /* TPT_OBJ_MESH: index into meshes */
+    int32_t pad_;
+    float center[3];
+    float radius;    /* sphere, circle */
+    float rot[9];    /* rectangle, circle: columns u, v, w (w = normal) */
+    float dim[2];    /* rectangle */
+} tpt_object;
+
+/* A triangle mesh, already in world space.  Each face holds 9 ints: for
+ * corners a, b, c the (vertex, normal, texcoord) indices, -1 when absent
+ * (tinyobj index_t, cpu/object.h:244-252).  Faces are grouped into `shapes`
+ * (each shape owned one BVHTree(faces, 4) in the reference, cpu/object.cpp:590-609). */
+typedef struct {
+    int32_t num_vertices, num_normals, num_texcoords, num_faces, num_shapes;
+    const float *vertices;        /* num_vertices * 3 */
+    const float *normals;         /* num_normals * 3 */
+    const float *texcoords;       /* num_texcoords * 2 */
+    const int32_t *faces;         /* num_faces * 9 */
+    const int32_t *face_material; /* num_faces, index into tpt_scene_desc.materials */
+    const int32_t *shape_offsets; /* num_shapes + 1 face offsets */
+} tpt_mesh;
+
+/* ---- lights (cpu/light.h:12-143) ------------------------------------------ */
+enum { TPT_LIGHT_DISTANT = 0, TPT_LIGHT_POINT = 1 };
+typedef struct {
+    int32_t type;
+    float rot[9];    /* distant: columns u, v, w; w points towards the light */
+    float center[3]; /* point */
+    float color[3];
+    float power;
+    float radius;    /* point: sphere radius; distant: angle (full cone angle, radians) */
+} tpt_delta_light;
+
+/* ---- camera (cpu/camera.h:14-79) and background (cpu/texture.h:83-109) ---- */
+typedef struct {
+    float center[3];
+    float rot[9];
+    int32_t width, height;
+    float fov, aperture, focus_dist;
+} tpt_camera;
+
+typedef struct {
+    tpt_texture rgb;
+    float rot[9];
+    /* 0: reference-compatible EnvTexture (its luminance CDF is never built, so
+     *    sampling is uniform in uv with pdf 1/(4 pi), texture.cpp:66-67);
+     * 1: build the max-channel CDF the reference intended (texture.cpp:68-76). */
+    int32_t importance;
+} tpt_background;
+
+typedef struct {
+    tpt_camera camera;
+    tpt_background background;
+    int32_t num_objects, num_meshes, num_materials, num_images, num_delta_lights;
+    const tpt_object *objects; /* top-level objects, in the order given to Scene() */
+    const tpt_mesh *meshes;
+    const tpt_material *materials;
+    const tpt_image *images;
+    const tpt_delta_light *delta_lights;
+} tpt_scene_desc;
+
+/* ---- render parameters ---------------------------------------------------- */
+enum { TPT_OUT_LDR_CLIPPED = 0, TPT_OUT_HDR_MEAN = 1 };
+typedef struct {
+    int32_t spp;            /* samples per pixel requested (integer-divided over sub-pixels) */
+    int32_t max_depth;      /* reference: 5 (cpu/path_tracer.cpp:103) */
+    int32_t rr_depth;       /* < 0: no Russian roulette (reference); else RR once depth >= rr_depth */
+    int32_t shadow_bounded; /* 0: reference unbounded delta-light shadow rays; 1: bounded by light distance */
+    uint64_t seed;          /* base of the per-(pixel, sample) erand48 streams */
+    int32_t output;         /* TPT_OUT_* */
+    int32_t accum_fp64;     /* accumulate samples in FP64 (convergence references) */
+    int32_t queue_capacity; /* max paths in flight; 0 = library default */
+    int32_t block_size;     /* threads per block for the wavefront kernels; 0 = default (256) */
+    int32_t count_traffic;  /* 1: count BVH node / triangle fetches in extend (slower; fills stats) */
+} tpt_render_params;
+
+typedef struct {
+    double render_ms;       /* wall time of the render, host clock */
+    double extend_ms;       /* summed device time of extend launches (HIP events) */
+    double shade_ms, shadow_ms, other_ms;
+    uint64_t paths;         /* paths traced = W * H * effective spp */
+    uint64_t rays_closest;  /* closest-hit queries */
+    uint64_t rays_shadow;   /* shadow (any-hit) queries */
+    uint64_t extend_launches;
+    uint64_t nodes_visited; /* BVH node fetches in extend, when counting is enabled */
+    uint64_t tris_tested;   /* triangle records fetched in extend, when counting is enabled */
+} tpt_render_stats;
+
+/* ---- opaque handle -------------------------------------------------------- */
+typedef struct tpt_hip_scene tpt_hip_scene;
+
+/* Library ABI version (TPT_ABI_VERSION). */
+int tpt_hip_abi_version(void);
+/* Number of visible HIP devices (0 when none); never throws. */
+int tpt_hip_device_count(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *tpt_hip_last_error(void);
+
+/* Scene::to(hip): build the BVH on the host, upload one device arena on the
+ * current device.  Replaces cuda::Scene(const cpu::Scene&) (cuda/scene.h:95-174). */
+int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out);
+/* Frees the device arena (cuda::Scene::~Scene, cuda/scene.h:176-197). */
+void tpt_hip_scene_destroy(tpt_hip_scene *scene);
+/* Size in bytes of the device arena and of its BVH parts. */
+int tpt_hip_scene_info(const tpt_hip_scene *scene, uint64_t *arena_bytes, uint64_t *num_nodes,
+                       uint64_t *num_triangles, uint64_t *num_analytic);
+
+/* PathTracer(hip).render(scene, spp): synchronous; writes H*W*3 floats (row 0
+ * at the bottom) into the HOST buffer `out_rgb`.  Replaces
+ * tinypt::PathTracer::render (path_tracer.h:26-44).  `stats` may be NULL. */
+int tpt_hip_render(tpt_hip_scene *scene, const tpt_render_params *params, float *out_rgb,
+                   tpt_render_stats *stats);
+
+/* Closest hit for n rays of 8 floats (org.xyz, tmin, dir.xyz, tmax); directions
+ * must be unit length as in the reference.  Writes the global primitive id
+ * (-1 on miss) and hit distance (INF on miss).  Geometric test only (no alpha
+ * cut-out).  `flags & TPT_PTR_DEVICE`: all pointers are device memory and the
+ * call is enqueued on `stream` (a hipStream_t, NULL = default) without a sync. */
+#define TPT_PTR_DEVICE 1u
+#define TPT_COUNT_TRAFFIC 2u
+int tpt_hip_trace_closest(tpt_hip_scene *scene, const float *rays, uint64_t n, int32_t *prim_id, float *t,
+                          uint32_t flags, void *stream);
+/* Any hit with tmin <= t < tmax: writes 1 when occluded, else 0. */
+int tpt_hip_trace_occluded(tpt_hip_scene *scene, const float *rays, uint64_t n, uint8_t *occluded, uint32_t flags,
+                           void *stream);
+/* Node / triangle fetch counters of the last TPT_COUNT_TRAFFIC trace call. */
+int tpt_hip_last_trace_counters(tpt_hip_scene *scene, uint64_t *nodes_visited, uint64_t *tris_tested);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TINYPT_HIP_H_ */

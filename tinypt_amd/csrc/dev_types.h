@@ -1,0 +1,177 @@
+// dev_types.h — POD layouts of the device arena, shared by the host scene
+// builder (scene_build.cpp) and the gfx950 kernels (kernels.hip).
+//
+// Arena layout (one hipMalloc per scene, every section 256-B aligned):
+//   nodes      BVH2 nodes, 64 B each (one 64-B line = both child boxes + refs)
+//   tris_bvh   triangles in BVH leaf order, 48 B each: (a.xyz, gid) (ab.xyz, tri_id) (ac.xyz, flags)
+//   tris       the same records in triangle-id order (surface / light-pdf lookups)
+//   tri_info   2 x int4 per triangle: (material, na, nb, nc) (ta, tb, tc, mesh)
+//   normals, texcoords, gid_local, analytic, materials, images, texels, lights, env CDF
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+namespace tpt {
+
+// bit casts for host code that fills device records
+inline float __int_as_float_host(int32_t i) {
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
+inline int32_t __float_as_int_host(float f) {
+    int32_t i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+
+static constexpr float kINF = 1e20f; // cpu/defs.h:23
+static constexpr float kEPS = 1e-3f; // cpu/defs.h:24
+
+// child reference inside a node: >= 0 inner node index; < 0 leaf
+//   leaf = 0x80000000 | (first_tri << 3) | (count - 1), count <= 8
+static constexpr uint32_t kLeafBit = 0x80000000u;
+static constexpr int kMaxLeaf = 8;
+static constexpr int kStackSize = 64;
+
+struct DTex {
+    int32_t image;   // -1: constant
+    int32_t channel; // first channel
+    float v[3];
+};
+
+struct DImage {
+    int32_t w, h, c, fmt; // fmt 0 = f32, 1 = u8
+    uint64_t offset;      // byte offset into the texel blob
+};
+
+struct DLobe {
+    int32_t type;
+    int32_t specular;
+    float param;  // ior / shininess
+    float alpha2; // GGX alpha^2 = 2/(s+2)
+    DTex color;
+};
+
+struct DMaterial {
+    DLobe lobe[2];
+    int32_t combine;
+    float weight;
+    int32_t specular;     // surface()->is_specular()
+    int32_t alpha_opaque; // AlphaTexture::empty()
+    DTex emission;
+    DTex alpha;
+    int32_t bump;         // gradient image or -1
+    int32_t emissive;
+};
+
+enum { A_SPHERE = 0, A_RECT = 1, A_CIRCLE = 2 };
+struct DAnalytic {
+    int32_t type, material, gid, pad;
+    float center[3];
+    float radius;
+    float u[3], v[3], w[3]; // frame columns
+    float dim[2];
+    float dist;
+    float pad2;
+};
+
+struct DDelta {
+    int32_t type; // 0 distant, 1 point
+    float u[3], v[3], w[3];
+    float center[3];
+    float emission[3]; // color * power
+    float radius;      // point radius
+    float angular_radius;
+};
+
+enum { L_ANALYTIC = 0, L_TRIANGLE = 1, L_ENV = 2 };
+struct DLight {
+    int32_t kind;
+    int32_t local; // analytic index / triangle id
+    int32_t gid;
+    int32_t pad;
+};
+
+struct DCamera {
+    float center[3];
+    float u[3], v[3], w[3];
+    float direct[3], right[3], up[3];
+    int32_t width, height;
+    float aperture;
+    float pad;
+};
+
+struct DScene {
+    const float4 *nodes;
+    const float4 *tris_bvh;
+    const float4 *tris;
+    const int4 *tri_info;
+    const float *normals;
+    const float *texcoords;
+    const int32_t *gid_local; // analytic: -(idx + 1); triangle: tri_id
+    const DAnalytic *analytic;
+    const DMaterial *materials;
+    const DImage *images;
+    const uint8_t *texels;
+    const DDelta *delta;
+    const DLight *lights;
+    const float *env_marg; // env_h + 1 floats
+    const float *env_cond; // env_h * (env_w + 1) floats
+    int32_t num_nodes;
+    int32_t root; // child reference of the root (inner 0, or a leaf when the BVH is one leaf), -1 if empty
+    int32_t num_tris;
+    int32_t num_analytic;
+    int32_t num_delta;
+    int32_t num_lights;
+    int32_t env_w, env_h;
+    int32_t bg_map;
+    int32_t pad0;
+    DTex bg;
+    float bg_rot[9]; // columns u, v, w
+    DCamera cam;
+};
+
+// Per-render parameters handed to the kernels.
+struct DRender {
+    uint64_t seed;
+    int32_t spp_eff;  // n_sub * b * b
+    int32_t block;    // b
+    int32_t nsub;
+    int32_t max_depth;
+    int32_t rr_depth;
+    int32_t shadow_bounded;
+    int32_t output;
+    int32_t accum_fp64;
+    int32_t pixels;   // W * H
+    int32_t k;        // samples per pixel in one wave
+};
+
+// Wavefront path state, SoA over slots (slot = s_local * pixels + pixel).
+struct DPaths {
+    float4 *org;   // xyz origin, w unused
+    float4 *dir;   // xyz direction
+    float4 *decay; // xyz throughput
+    float4 *color; // xyz radiance
+    uint64_t *rng; // 48-bit erand48 state
+    uint64_t *key; // stream seed (alpha hash key)
+    int32_t *depth;
+    // queues
+    int32_t *queue_a;
+    int32_t *queue_b;
+    int32_t *shadow_slot;
+    float4 *shadow_org;  // xyz, w = tmax
+    float4 *shadow_dir;  // xyz
+    float4 *shadow_val;  // contribution
+    float4 *hit;         // t, gid(bits), u, v — indexed by queue position
+    uint32_t *counters;  // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] traffic counters
+    // per pixel accumulation
+    float *sub_f;   // 3 per pixel: running sub-pixel sum (fp32)
+    double *sub_d;  // 3 per pixel (fp64 mode)
+    float *pix_f;   // 3 per pixel: sum of clipped sub-pixel means (or hdr sum)
+    double *pix_d;  // 3 per pixel (fp64 hdr)
+};
+
+} // namespace tpt

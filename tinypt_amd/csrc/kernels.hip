@@ -1,0 +1,566 @@
+// kernels.hip — gfx950 wavefront path tracer (device=hip).
+//
+// Stages per wave of paths (all state SoA in HBM, queues compacted with wave64
+// ballot + one atomic per wavefront):
+//   generate   pixel/sample -> camera ray + erand48 state        (cpu/path_tracer.cpp:77-96, camera.cpp:23-34)
+//   extend     closest hit over analytic prims + BVH2             (cpu/scene.h:27, object.cpp:389-411, bvh.cpp:78-118)
+//   shade      emission, BxDF/light mixture sampling, delta-light
+//              shadow-ray emission, throughput update, RR         (cpu/path_tracer.cpp:102-168)
+//   shadow     any-hit for delta-light shadow rays                (cpu/path_tracer.cpp:129-139)
+//   accumulate per-pixel sub-pixel sums in sample order           (cpu/path_tracer.cpp:80-96)
+//   finalize   clip(sum / b^2) into the output image
+#include <hip/hip_runtime.h>
+
+#include "dev_scene.h"
+
+namespace tpt {
+
+// ---------------------------------------------------------------- traversal
+// Child box slab test.  The GPU BVH is this library's own tree, so the test
+// only has to be conservative: a zero direction component uses a large finite
+// reciprocal (no 0*inf NaNs) and the far distance is widened by 2^-17
+// relative so that a triangle whose exact Moller-Trumbore t lies on a box face
+// is never culled.
+__device__ __forceinline__ float safe_rcp(float d) {
+    return (fabsf(d) > 1e-30f) ? 1.f / d : copysignf(1e30f, d);
+}
+__device__ __forceinline__ bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, V3 o, V3 inv,
+                                        float tmax, float &tnear) {
+    float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+    float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+    float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    f = f * 1.0000077f;
+    tnear = n;
+    return n <= f && f >= kEPS * 0.99f && n <= tmax;
+}
+
+struct AlphaCtx {
+    uint64_t key;
+    int depth;
+    bool on;
+};
+
+struct HitRec {
+    float t;
+    int32_t gid;
+    float u, v; // triangle barycentrics; analytic: u = into
+};
+
+// alpha test of a triangle candidate (object.cpp:273-287)
+__device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u, float v, const AlphaCtx &ac,
+                                               int32_t gid) {
+    int4 i0 = S.tri_info[2 * tri], i1 = S.tri_info[2 * tri + 1];
+    const DMaterial &m = S.materials[i0.x];
+    V2 uv;
+    if (i1.x >= 0) {
+        const float *tc = S.texcoords;
+        float w0 = 1.f - u - v;
+        uv = {(w0 * tc[2 * i1.x] + u * tc[2 * i1.y]) + v * tc[2 * i1.z],
+              (w0 * tc[2 * i1.x + 1] + u * tc[2 * i1.y + 1]) + v * tc[2 * i1.z + 1]};
+    } else {
+        uv = {u, v};
+    }
+    return !(alpha_at(S, m.alpha, uv) < alpha_draw(ac.key, ac.depth, gid));
+}
+
+// Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
+// Ties on t resolve to the smaller global primitive id.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
+                                        uint32_t &tris) {
+    HitRec best{tmax, -1, 0.f, 0.f};
+    for (int i = 0; i < S.num_analytic; i++) {
+        const DAnalytic &p = S.analytic[i];
+        float t;
+        bool into = true;
+        if (p.type == A_SPHERE) {
+            t = sphere_t(p, o, d, into);
+        } else {
+            V2 uv;
+            t = plane_t(p, o, d, kINF, uv);
+        }
+        if (t < kINF && (t < best.t || (t == best.t && p.gid < best.gid))) {
+            if (!ANY && ac.on && p.type == A_SPHERE && !S.materials[p.material].alpha_opaque) {
+                V3 n = nrm(sub(at(o, d, t), mk(p.center)));
+                V2 uv{(atan2f(n.y, n.x) + kPI) * (.5f * kINV_PI), acosf(n.z) * kINV_PI};
+                if (alpha_at(S, S.materials[p.material].alpha, uv) < alpha_draw(ac.key, ac.depth, p.gid)) continue;
+            }
+            best = HitRec{t, p.gid, into ? 1.f : 0.f, 0.f};
+            if (ANY) return best;
+        }
+    }
+    if (S.root == -1) return best;
+    V3 inv{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+    int32_t stack[kStackSize];
+    int sp = 0;
+    int32_t cur = S.root;
+    while (true) {
+        if (cur >= 0) {
+            const float4 *nd = S.nodes + 4 * cur;
+            float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+            if (COUNT) nodes++;
+            float n0, n1;
+            bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, best.t, n0);
+            bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, best.t, n1);
+            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+            if (h0 && h1) {
+                int32_t nearc = (n1 < n0) ? c1 : c0;
+                int32_t farc = (n1 < n0) ? c0 : c1;
+                stack[sp++] = farc;
+                cur = nearc;
+                continue;
+            }
+            if (h0) {
+                cur = c0;
+                continue;
+            }
+            if (h1) {
+                cur = c1;
+                continue;
+            }
+        } else {
+            uint32_t ref = (uint32_t)cur;
+            int first = (int)((ref & ~kLeafBit) >> 3);
+            int cnt = (int)(ref & 7u) + 1;
+            for (int k = 0; k < cnt; k++) {
+                const float4 *tr = S.tris_bvh + 3 * (first + k);
+                float4 r0 = tr[0], r1 = tr[1], r2 = tr[2];
+                if (COUNT) tris++;
+                float t, u, v;
+                if (!tri_test(mk4(r0), mk4(r1), mk4(r2), o, d, t, u, v)) continue;
+                int32_t gid = __float_as_int(r0.w);
+                if (kEPS <= t && (t < best.t || (t == best.t && gid < best.gid))) {
+                    if (!ANY && ac.on && (__float_as_int(r2.w) & 1)) {
+                        if (!tri_alpha_pass(S, __float_as_int(r1.w), u, v, ac, gid)) continue;
+                    }
+                    best = HitRec{t, gid, u, v};
+                    if (ANY) return best;
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack[--sp];
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------- wave-level compaction helpers
+__device__ __forceinline__ int wave_append(bool pred, uint32_t *counter) {
+    uint64_t mask = __ballot(pred);
+    int lane = (int)__lane_id();
+    int total = __popcll(mask);
+    int prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    if (total > 0 && lane == leader) base = atomicAdd(counter, (uint32_t)total);
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return pred ? (int)base + prefix : -1;
+}
+
+__device__ __forceinline__ void wave_count(uint32_t v32, uint32_t *counter) {
+    // 64-bit counter at `counter` (8-B aligned); reduce within the wave, one atomic per wave
+    unsigned long long v = v32;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long *>(counter), v);
+}
+
+// ---------------------------------------------------------------- generate
+__global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_paths, uint64_t base_mixed) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) P.counters[0] = (uint32_t)n_paths;
+    if (j >= n_paths) return;
+    int pix = j % R.pixels;
+    int s = wave_base + j / R.pixels;
+    int W = S.cam.width;
+    int x = pix % W, y = pix / W;
+    uint64_t key = stream_seed(base_mixed, (uint64_t)pix, (uint64_t)s);
+    Rng rng{rng_state_from_seed(key)};
+    // PixelSampler sub-pixel origin (path_tracer.cpp:8-58): repeated additions of 1/b
+    int subpix = s / R.nsub;
+    int sx = subpix % R.block, sy = subpix / R.block;
+    float w = 1.f / (float)R.block;
+    float px = (float)x - .5f, py = (float)y - .5f;
+    for (int i = 0; i < sx; i++) px += w;
+    for (int i = 0; i < sy; i++) py += w;
+    px = px + rng.random(w);
+    py = py + rng.random(w);
+    // Camera::shoot_ray (camera.cpp:23-34)
+    const DCamera &c = S.cam;
+    V3 offset{0.f, 0.f, 0.f};
+    if (c.aperture >= kEPS) {
+        V2 uo = rng.rand_in_disk(c.aperture);
+        offset = add(scl(uo.x, mk(c.u)), scl(uo.y, mk(c.v)));
+    }
+    float a = px / (float)c.width - .5f, b = py / (float)c.height - .5f;
+    V3 dir = sub(add(add(scl(a, mk(c.right)), scl(b, mk(c.up))), mk(c.direct)), offset);
+    dir = nrm(dir);
+    V3 org = add(mk(c.center), offset);
+    P.org[j] = make_float4(org.x, org.y, org.z, 0.f);
+    P.dir[j] = make_float4(dir.x, dir.y, dir.z, 0.f);
+    P.decay[j] = make_float4(1.f, 1.f, 1.f, 0.f);
+    P.color[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    P.rng[j] = rng.x;
+    P.key[j] = key;
+    P.depth[j] = 0;
+    P.queue_a[j] = j;
+}
+
+// ---------------------------------------------------------------- extend
+template <bool COUNT>
+__global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
+    uint32_t n = P.counters[qsel];
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        P.counters[nsel_next] = 0u; // next ray queue
+        P.counters[2] = 0u;         // shadow queue
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 8), (unsigned long long)n);
+    }
+    uint32_t nodes = 0, tris = 0;
+    for (uint32_t i = (uint32_t)tid; i < n; i += gridDim.x * blockDim.x) {
+        int slot = queue[i];
+        float4 o4 = P.org[slot], d4 = P.dir[slot];
+        AlphaCtx ac{P.key[slot], P.depth[slot], true};
+        HitRec h = trace<false, COUNT>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris);
+        P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+    }
+    if (COUNT) {
+        wave_count(nodes, P.counters + 4);
+        wave_count(tris, P.counters + 6);
+    }
+}
+
+// ---------------------------------------------------------------- shade
+__device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRec &h, SurfHit &sh, int &mat) {
+    int32_t loc = S.gid_local[h.gid];
+    if (loc < 0) {
+        const DAnalytic &p = S.analytic[-loc - 1];
+        mat = p.material;
+        if (p.type == A_SPHERE) { // object.cpp:44-59
+            V3 pos = at(o, d, h.t);
+            V3 n = nrm(sub(pos, mk(p.center)));
+            float u = (atan2f(n.y, n.x) + kPI) * (.5f * kINV_PI);
+            float v = acosf(n.z) * kINV_PI;
+            bool into = h.u != 0.f;
+            if (!into) n = neg(n);
+            sh = SurfHit{n, n, V2{u, v}, into};
+        } else { // object.cpp:123-126, 170-174
+            V2 uv;
+            plane_t(p, o, d, kINF, uv);
+            V3 n = mk(p.w);
+            V3 hn = same_side(n, d) ? neg(n) : n;
+            sh = SurfHit{hn, hn, uv, true};
+        }
+        return;
+    }
+    // triangle (object.cpp:266-331)
+    int tri = loc;
+    int4 i0 = S.tri_info[2 * tri], i1 = S.tri_info[2 * tri + 1];
+    mat = i0.x;
+    float u = h.u, v = h.v;
+    float4 r1 = S.tris[3 * tri + 1], r2 = S.tris[3 * tri + 2];
+    V3 ab = mk4(r1), ac = mk4(r2);
+    V2 uv;
+    const float *tc = S.texcoords;
+    if (i1.x >= 0) {
+        float w0 = 1.f - u - v;
+        uv = {(w0 * tc[2 * i1.x] + u * tc[2 * i1.y]) + v * tc[2 * i1.z],
+              (w0 * tc[2 * i1.x + 1] + u * tc[2 * i1.y + 1]) + v * tc[2 * i1.z + 1]};
+    } else {
+        uv = {u, v};
+    }
+    V3 n;
+    if (i0.y >= 0) {
+        const float *nm = S.normals;
+        float w0 = 1.f - u - v;
+        V3 na = mk(nm + 3 * i0.y), nb = mk(nm + 3 * i0.z), nc = mk(nm + 3 * i0.w);
+        n = nrm(add(add(scl(w0, na), scl(u, nb)), scl(v, nc)));
+    } else {
+        n = nrm(cross(ac, ab));
+    }
+    bool into = !same_side(n, d);
+    if (!into) n = neg(n);
+    V3 sn = n;
+    const DMaterial &m = S.materials[mat];
+    if (m.bump >= 0) {
+        V2 dab, dac;
+        if (i1.x >= 0) {
+            dab = {tc[2 * i1.y] - tc[2 * i1.x], tc[2 * i1.y + 1] - tc[2 * i1.x + 1]};
+            dac = {tc[2 * i1.z] - tc[2 * i1.x], tc[2 * i1.z + 1] - tc[2 * i1.x + 1]};
+        } else {
+            dab = {1.f, 0.f};
+            dac = {0.f, 1.f};
+        }
+        float inv_det = 1.f / (dab.x * dac.y - dac.x * dab.y);
+        if (isfinite(inv_det)) {
+            V3 dpdu = scr(sub(scl(dac.y, ab), scl(dab.y, ac)), inv_det);
+            V3 dpdv = scr(add(scl(-dac.x, ab), scl(dab.x, ac)), inv_det);
+            sn = bump_normal(S, m.bump, uv, sn, dpdu, dpdv);
+        }
+    }
+    sh = SurfHit{n, sn, uv, into};
+}
+
+__global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel) {
+    uint32_t n = P.counters[qsel];
+    const float LIGHT_PROB = 1.f / 5.f; // cpu/path_tracer.cpp:104
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < n;
+        bool alive = false;
+        bool emit_shadow = false;
+        int slot = 0;
+        V3 s_org{0, 0, 0}, s_dir{0, 0, 0}, s_val{0, 0, 0};
+        float s_tmax = kINF;
+        if (valid) {
+            slot = queue[i];
+            float4 h4 = P.hit[i];
+            HitRec h{h4.x, __float_as_int(h4.y), h4.z, h4.w};
+            V3 o = mk4(P.org[slot]), d = mk4(P.dir[slot]);
+            V3 decay = mk4(P.decay[slot]);
+            V3 color = mk4(P.color[slot]);
+            int depth = P.depth[slot];
+            Rng rng{P.rng[slot]};
+            if (!(h.t < kINF)) {
+                color = add(color, cmul(decay, background(S, d)));
+            } else {
+                SurfHit sh;
+                int mi;
+                surface(S, o, d, h, sh, mi);
+                const DMaterial &m = S.materials[mi];
+                color = add(color, cmul(decay, color_at(S, m.emission, sh.uv)));
+                if (depth < R.max_depth) {
+                    V3 hp = at(o, d, h.t);
+                    float pdf_bxdf;
+                    bool spec;
+                    int sel = sample_bxdf(m, rng, pdf_bxdf, spec);
+                    V3 out, albedo;
+                    if (!spec) {
+                        if (S.num_delta > 0) {
+                            const DDelta &dl = S.delta[rng.rand_int(S.num_delta)];
+                            float tmax;
+                            V3 tl = delta_sample(dl, hp, rng, tmax);
+                            if (!R.shadow_bounded) tmax = kINF;
+                            V3 la = bxdf_f(S, m, sel, d, sh, tl);
+                            float pdf_light = 1.f / (float)S.num_delta;
+                            V3 contrib = dvs(cmul(cmul(decay, la), mk(dl.emission)), pdf_light);
+                            if (contrib.x != 0.f || contrib.y != 0.f || contrib.z != 0.f) {
+                                emit_shadow = true;
+                                s_org = hp;
+                                s_dir = tl;
+                                s_val = contrib;
+                                s_tmax = tmax;
+                            }
+                        }
+                        float pdf_val;
+                        if (S.num_lights > 0) {
+                            if (rng.random() < LIGHT_PROB) {
+                                out = lights_sample(S, hp, rng);
+                            } else {
+                                out = bxdf_sample(m, sel, d, sh, rng);
+                            }
+                            pdf_val = (1.f - LIGHT_PROB) * bxdf_pdf(m, sel, d, sh, out) +
+                                      LIGHT_PROB * lights_pdf(S, hp, out);
+                        } else {
+                            out = bxdf_sample(m, sel, d, sh, rng);
+                            pdf_val = bxdf_pdf(m, sel, d, sh, out);
+                        }
+                        albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / ((pdf_val + kEPS) * pdf_bxdf));
+                    } else {
+                        out = bxdf_sample(m, sel, d, sh, rng);
+                        albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / pdf_bxdf);
+                    }
+                    decay = cmul(decay, albedo);
+                    alive = !is_zero5(decay);
+                    if (alive && R.rr_depth >= 0 && depth >= R.rr_depth) {
+                        float mx = fmaxf(decay.x, fmaxf(decay.y, decay.z));
+                        float q = fminf(mx, 0.95f);
+                        if (rng.random() >= q) {
+                            alive = false;
+                        } else {
+                            decay = dvs(decay, q);
+                        }
+                    }
+                    if (alive) {
+                        P.org[slot] = make_float4(hp.x, hp.y, hp.z, 0.f);
+                        P.dir[slot] = make_float4(out.x, out.y, out.z, 0.f);
+                        P.decay[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
+                        P.depth[slot] = depth + 1;
+                    }
+                }
+            }
+            P.color[slot] = make_float4(color.x, color.y, color.z, 0.f);
+            P.rng[slot] = rng.x;
+        }
+        int pos = wave_append(alive, P.counters + nsel);
+        if (alive) next[pos] = slot;
+        int spos = wave_append(emit_shadow, P.counters + 2);
+        if (emit_shadow) {
+            P.shadow_slot[spos] = slot;
+            P.shadow_org[spos] = make_float4(s_org.x, s_org.y, s_org.z, s_tmax);
+            P.shadow_dir[spos] = make_float4(s_dir.x, s_dir.y, s_dir.z, 0.f);
+            P.shadow_val[spos] = make_float4(s_val.x, s_val.y, s_val.z, 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- shadow
+template <bool COUNT>
+__global__ void k_shadow(DScene S, DPaths P) {
+    uint32_t n = P.counters[2];
+    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 10), (unsigned long long)n);
+    uint32_t nodes = 0, tris = 0;
+    AlphaCtx ac{0, 0, false};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 o4 = P.shadow_org[i], d4 = P.shadow_dir[i];
+        HitRec h = trace<true, false>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris);
+        if (h.gid < 0) {
+            int slot = P.shadow_slot[i];
+            float4 c = P.color[slot], v = P.shadow_val[i];
+            P.color[slot] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, 0.f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- accumulate / finalize
+__global__ void k_accumulate(DRender R, DPaths P, int wave_base, int k_active, int first_wave) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= R.pixels) return;
+    float sf[3], pf[3];
+    double sd[3], pd[3];
+    for (int c = 0; c < 3; c++) {
+        sf[c] = first_wave ? 0.f : P.sub_f[3 * p + c];
+        pf[c] = first_wave ? 0.f : P.pix_f[3 * p + c];
+        sd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.sub_d[3 * p + c];
+        pd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.pix_d[3 * p + c];
+    }
+    for (int j = 0; j < k_active; j++) {
+        int s = wave_base + j;
+        float4 L = P.color[(size_t)j * R.pixels + p];
+        if (R.accum_fp64) {
+            sd[0] += L.x;
+            sd[1] += L.y;
+            sd[2] += L.z;
+        } else {
+            sf[0] = sf[0] + L.x;
+            sf[1] = sf[1] + L.y;
+            sf[2] = sf[2] + L.z;
+        }
+        if ((s + 1) % R.nsub == 0) { // sub-pixel complete (path_tracer.cpp:91)
+            for (int c = 0; c < 3; c++) {
+                if (R.accum_fp64) {
+                    if (R.output == 1) pd[c] += sd[c];
+                    else pf[c] = pf[c] + clip1((float)(sd[c] / R.nsub));
+                    sd[c] = 0.0;
+                } else {
+                    if (R.output == 1) pf[c] = pf[c] + sf[c];
+                    else pf[c] = pf[c] + clip1(sf[c] / (float)R.nsub);
+                    sf[c] = 0.f;
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        P.sub_f[3 * p + c] = sf[c];
+        P.pix_f[3 * p + c] = pf[c];
+        if (R.accum_fp64) {
+            P.sub_d[3 * p + c] = sd[c];
+            P.pix_d[3 * p + c] = pd[c];
+        }
+    }
+}
+
+__global__ void k_finalize(DRender R, DPaths P, float *out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= R.pixels) return;
+    int bb = R.block * R.block;
+    for (int c = 0; c < 3; c++) {
+        float v;
+        if (R.output == 1) {
+            if (R.accum_fp64) v = (float)(P.pix_d[3 * p + c] / ((double)R.nsub * (double)bb));
+            else v = P.pix_f[3 * p + c] / (float)(R.nsub * bb);
+        } else {
+            v = clip1(P.pix_f[3 * p + c] / (float)bb);
+        }
+        out[3 * p + c] = v;
+    }
+}
+
+// ---------------------------------------------------------------- ray queries (parity / bench)
+template <bool COUNT>
+__global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_t *prim, float *tout,
+                                uint32_t *counters) {
+    uint32_t nodes = 0, tris = 0;
+    AlphaCtx ac{0, 0, false};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        float tmax = b.w < kINF ? b.w : kINF;
+        HitRec h = trace<false, COUNT>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        bool hit = h.gid >= 0;
+        prim[i] = hit ? h.gid : -1;
+        tout[i] = hit ? h.t : kINF;
+    }
+    if (COUNT) {
+        wave_count(nodes, counters + 0);
+        wave_count(tris, counters + 2);
+    }
+}
+
+__global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8_t *occ) {
+    uint32_t nodes = 0, tris = 0;
+    AlphaCtx ac{0, 0, false};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        float4 a = rays[2 * i], b = rays[2 * i + 1];
+        float tmax = b.w < kINF ? b.w : kINF;
+        HitRec h = trace<true, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        occ[i] = h.gid >= 0 ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------- host-visible launchers
+void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
+                     uint64_t base_mixed, int block, hipStream_t st) {
+    int grid = (n_paths + block - 1) / block;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+}
+void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int qsel, int nsel, int grid, int block,
+                   bool count, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
+}
+void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int32_t *queue, int32_t *next, int qsel,
+                  int nsel, int grid, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel);
+}
+void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(block), 0, st, S, P);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(block), 0, st, S, P);
+}
+void launch_accumulate(const DRender &R, const DPaths &P, int wave_base, int k_active, int first, hipStream_t st) {
+    int grid = (R.pixels + 255) / 256;
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, st, R, P, wave_base, k_active, first);
+}
+void launch_finalize(const DRender &R, const DPaths &P, float *out, hipStream_t st) {
+    int grid = (R.pixels + 255) / 256;
+    hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(256), 0, st, R, P, out);
+}
+void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32_t *prim, float *t,
+                          uint32_t *counters, bool count, int grid, int block, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(block), 0, st, S, rays, n, prim, t, counters);
+    else
+        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(block), 0, st, S, rays, n, prim, t, counters);
+}
+void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint8_t *occ, int grid, int block,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_trace_occluded, dim3(grid), dim3(block), 0, st, S, rays, n, occ);
+}
+
+} // namespace tpt

@@ -1,0 +1,235 @@
+"""Seeded scene generators for the benchmark workloads, built through the
+tinypt-style object API of this package.
+
+* ``cornell_spheres`` — the analytic two-spheres-in-a-box scene: six radius-1e5
+  wall spheres (two tinted, four white diffuse), a mirror and a glass sphere
+  (r = 16.5, IOR 1.5) and an emissive r = 600 light sphere (emission 12), laid
+  out like smallpt / tinypt's ``make_debug2`` (src/examples/scene_factory.cpp:225-291).
+* ``triangle_room`` — an axis-aligned box room plus 200 instanced meshes
+  (icospheres and subdivided boxes, random rotation, scale U(0.2, 2)), baked
+  into ONE world-space TriangleMesh of ~265k triangles; materials 60 %
+  Lambertian / 15 % metal / 10 % dielectric / 15 % GGX; four quad area lights.
+
+The reference has no API to build meshes from arrays (only ``from_obj``);
+``TriangleMesh.from_arrays`` stands in for an OBJ with one shape.
+"""
+import math
+
+import numpy as np
+
+from . import _C
+
+
+def rgb(v):
+    return _C.RGBTexture([float(x) for x in v])
+
+
+def material(surface, emission=None):
+    m = _C.Material(surface)
+    if emission is not None:
+        m.emission_texture = rgb(emission)
+    return m
+
+
+def lookat_rotation(forward, up=(0.0, 0.0, 1.0)):
+    """Camera rotation with columns (u, v, w), w = -forward (cpu/camera.cpp:17)."""
+    f = np.asarray(forward, dtype=np.float64)
+    f = f / np.linalg.norm(f)
+    u = np.cross(f, np.asarray(up, dtype=np.float64))
+    u = u / np.linalg.norm(u)
+    w = -f
+    v = np.cross(w, u)
+    return np.stack([u, v, w], axis=1).astype(np.float32)
+
+
+# ---------------------------------------------------------------- cornell spheres
+def cornell_spheres(width=512, height=384):
+    def sphere(center, radius, surface, emission=None):
+        s = _C.Sphere()
+        s.location = [float(c) for c in center]
+        s.radius = float(radius)
+        s.material = material(surface, emission)
+        return s
+
+    white = _C.Lambertian(rgb((0.75, 0.75, 0.75)))
+    objs = [
+        sphere((1e5 + 1, 40.8, 81.6), 1e5, _C.Lambertian(rgb((0.75, 0.25, 0.25)))),   # left
+        sphere((-1e5 + 99, 40.8, 81.6), 1e5, _C.Lambertian(rgb((0.25, 0.25, 0.75)))),  # right
+        sphere((50, 40.8, 1e5), 1e5, white),                                           # back
+        sphere((50, 40.8, -1e5 + 170), 1e5, white),                                    # front
+        sphere((50, 1e5, 81.6), 1e5, white),                                           # bottom
+        sphere((50, -1e5 + 81.6, 81.6), 1e5, white),                                   # top
+        sphere((27, 16.5, 47), 16.5, _C.Metal(rgb((0.999, 0.999, 0.999)))),
+        sphere((73, 16.5, 78), 16.5, _C.Dielectric(rgb((0.999, 0.999, 0.999)), 1.5)),
+        sphere((50, 681.6 - 0.27, 81.6), 600, _C.Lambertian(rgb((0, 0, 0))), emission=(12, 12, 12)),
+    ]
+    cam = _C.Camera()
+    # smallpt's camera at z = 295.6 starts its rays 140 units forward (inside the
+    # front wall); tinypt's pinhole has no near plane, so the eye sits at z = 160.
+    cam.location = [50.0, 52.0, 160.0]
+    w = np.array([0.0, 0.042612, 1.0])
+    w /= np.linalg.norm(w)
+    u = np.array([1.0, 0.0, 0.0])
+    v = np.cross(w, u)
+    cam.rotation = np.stack([u, v, w], axis=1).astype(np.float32)
+    cam.width, cam.height = width, height
+    cam.fov = math.radians(55.0)
+    return _C.Scene(cam, objs, [], None)
+
+
+# ---------------------------------------------------------------- meshes
+def icosphere(level):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(level):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts)
+    return V, np.array(faces, dtype=np.int64), V.copy()
+
+
+def subdivided_box(n):
+    verts, faces = [], []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            base = len(verts)
+            a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+            for i in range(n + 1):
+                for j in range(n + 1):
+                    p = [0.0, 0.0, 0.0]
+                    p[axis] = sign
+                    p[a1] = -1 + 2 * i / n
+                    p[a2] = -1 + 2 * j / n
+                    verts.append(p)
+            for i in range(n):
+                for j in range(n):
+                    q00 = base + i * (n + 1) + j
+                    q10, q01, q11 = q00 + n + 1, q00 + 1, q00 + n + 2
+                    if sign > 0:
+                        faces += [(q00, q10, q11), (q00, q11, q01)]
+                    else:
+                        faces += [(q00, q11, q10), (q00, q01, q11)]
+    return np.array(verts, dtype=np.float64), np.array(faces, dtype=np.int64)
+
+
+def random_rotation(rng):
+    u1, u2, u3 = rng.random(3)
+    q = np.array([math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                  math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3)])
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def room_materials(rng, n):
+    """Per-instance materials: 60% Lambertian U(0.1,0.9), 15% metal, 10% dielectric IOR U(1.3,1.7),
+    15% GGX roughness U(0.05,0.6) (alpha = roughness, shininess = 2/alpha^2 - 2)."""
+    mats = []
+    for _ in range(n):
+        r = rng.random()
+        if r < 0.60:
+            mats.append(material(_C.Lambertian(rgb(rng.uniform(0.1, 0.9, 3)))))
+        elif r < 0.75:
+            mats.append(material(_C.Metal(rgb(rng.uniform(0.5, 0.95, 3)))))
+        elif r < 0.85:
+            mats.append(material(_C.Dielectric(rgb((0.999, 0.999, 0.999)), float(rng.uniform(1.3, 1.7)))))
+        else:
+            rough = float(rng.uniform(0.05, 0.6))
+            shin = 2.0 / (rough * rough) - 2.0
+            mats.append(material(_C.Glossy(shin, rgb(rng.uniform(0.5, 0.95, 3)), _C.Glossy.MICRO_TROWBRIDGE_REITZ)))
+    return mats
+
+
+def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150):
+    """World-space (vertices, normals-or-None, faces9, face_material, materials-spec) for the room."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = 40.0, 40.0, 20.0
+    # room box (inward facing), no vertex normals
+    rv = np.array([[0, 0, 0], [X, 0, 0], [X, Y, 0], [0, Y, 0], [0, 0, Z], [X, 0, Z], [X, Y, Z], [0, Y, Z]], dtype=np.float64)
+    rf = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4], [3, 7, 6], [3, 6, 2],
+                   [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], dtype=np.int64)
+    ico_v, ico_f, _ = icosphere(ico_level)
+    box_v, box_f = subdivided_box(box_n)
+    kinds = np.array([0] * n_ico + [1] * (instances - n_ico))
+    rng.shuffle(kinds)
+    V = [rv]
+    N = [np.zeros_like(rv)]
+    F9 = []
+    FM = []
+    # room faces: material 0, no normals
+    for f in rf:
+        F9.append([f[0], -1, -1, f[1], -1, -1, f[2], -1, -1])
+        FM.append(0)
+    nv = len(rv)
+    nn = len(rv)
+    for i in range(instances):
+        R = random_rotation(rng)
+        s = float(rng.uniform(0.2, 2.0))
+        pos = np.array([rng.uniform(3, X - 3), rng.uniform(6, Y - 3), rng.uniform(0.5, 12.0)])
+        if kinds[i] == 0:
+            lv, lf, ln = ico_v, ico_f, ico_v
+        else:
+            lv, lf, ln = box_v, box_f, None
+        wv = (lv * s) @ R.T + pos
+        V.append(wv)
+        if ln is not None:
+            N.append(ln @ R.T)
+        for f in lf:
+            if ln is not None:
+                F9.append([f[0] + nv, f[0] + nn, -1, f[1] + nv, f[1] + nn, -1, f[2] + nv, f[2] + nn, -1])
+            else:
+                F9.append([f[0] + nv, -1, -1, f[1] + nv, -1, -1, f[2] + nv, -1, -1])
+            FM.append(1 + i)
+        nv += len(lv)
+        if ln is not None:
+            nn += len(ln)
+    verts = np.concatenate(V).astype(np.float32)
+    normals = np.concatenate(N).astype(np.float32)
+    faces9 = np.array(F9, dtype=np.int32)
+    fm = np.array(FM, dtype=np.int32)
+    return verts, normals, faces9, fm, rng
+
+
+def triangle_room(width=1024, height=768, seed=1):
+    verts, normals, faces9, fm, rng = triangle_room_arrays(seed)
+    mats = [material(_C.Lambertian(rgb((0.75, 0.75, 0.75))))] + room_materials(rng, 200)
+    mesh = _C.TriangleMesh.from_arrays(verts, faces9, mats, fm, normals, None)
+    lights = []
+    for cx, cy in ((10, 10), (30, 10), (10, 30), (30, 30)):
+        r = _C.Rectangle()
+        r.location = [float(cx), float(cy), 19.9]
+        r.dimension = [6.0, 6.0]
+        e = float(rng.uniform(5, 20))
+        r.material = material(_C.Lambertian(rgb((0, 0, 0))), emission=(e, e, e))
+        lights.append(r)
+    cam = _C.Camera()
+    cam.location = [20.0, 1.0, 9.0]
+    cam.rotation = lookat_rotation((0.0, 1.0, -0.15))
+    cam.width, cam.height = width, height
+    cam.fov = math.radians(65.0)
+    return _C.Scene(cam, [mesh] + lights, [], None)
+
+
+# presets: (spp, max_depth, rr_depth)
+CONFIG0 = dict(width=512, height=384, spp=64, max_depth=10, rr_depth=5)
+CONFIG1 = dict(width=1024, height=768, spp=256, max_depth=8, rr_depth=5)
