@@ -25,7 +25,21 @@ HBM_PEAK_GBS = 8000.0
 RAY_BYTES, NODE_BYTES, TRI_BYTES = 52, 64, 48
 
 
-def cpu_baseline(spp=16, rows=48, width=1024, height=768):
+def copy_calibration(mib=1024):
+    """A plain device-to-device copy of known size through torch (for scaling the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters of the extend kernel)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
+
+
+def cpu_baseline(spp=64, rows=256, width=1024, height=768):
     """The CPU oracle (C++/OpenMP restatement of the reference CPU renderer) on a
     bounded strip of the same workload: `rows` rows of the 1024x768 image."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -65,7 +79,11 @@ def main():
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path would shard by sample index with an RCCL reduce of the accumulation buffer)",
