@@ -94,6 +94,22 @@ def test_occluded_matches_oracle(room, hip):
     assert len(diff) <= 10
 
 
+def test_binary_layout_matches_oracle(hip, monkeypatch):
+    # TPT_BVH_WIDTH=2 builds the binary 64-B node layout instead of the 4-wide one
+    from tinypt_amd import scenes
+
+    monkeypatch.setenv("TPT_BVH_WIDTH", "2")
+    host = scenes.triangle_room(64, 48)
+    dev = host.to("hip")
+    oracle = O.OracleScene(host)
+    rays = raysets.extend_set(oracle, _room_cam(), 100_000, seed=21)
+    gp, gt = hip.trace_closest(dev, rays)
+    op, ot = oracle.trace_closest(rays, mode=0)
+    mism, ties, prune, bad = _classify(oracle, rays, gp, gt, op, ot)
+    assert not bad and len(mism) <= 20
+    assert np.array_equal(gt[gp == op], ot[gp == op])
+
+
 def test_spheres_closest_hit(hip):
     from tinypt_amd import scenes
 

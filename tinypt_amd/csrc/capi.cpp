@@ -140,7 +140,8 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.lights = (const DLight *)(base + o_li);
     S.env_marg = (const float *)(base + o_em);
     S.env_cond = (const float *)(base + o_ec);
-    S.num_nodes = (int32_t)(H.nodes.size() / 4);
+    S.wide = H.wide;
+    S.num_nodes = (int32_t)(H.nodes.size() / (H.wide ? 8 : 4));
     S.root = H.root;
     S.num_tris = (int32_t)(H.tris_bvh.size() / 3);
     S.num_analytic = (int32_t)H.analytic.size();

@@ -2,7 +2,8 @@
 // builder (scene_build.cpp) and the gfx950 kernels (kernels.hip).
 //
 // Arena layout (one hipMalloc per scene, every section 256-B aligned):
-//   nodes      BVH2 nodes, 64 B each (one 64-B line = both child boxes + refs)
+//   nodes      BVH4 nodes, 128 B each: SoA child boxes lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4],
+//              child refs[4], pad  (or, when built binary, BVH2 nodes of 64 B: both child boxes + refs)
 //   tris_bvh   triangles in BVH leaf order, 48 B each: (a.xyz, gid) (ab.xyz, tri_id) (ac.xyz, flags)
 //   tris       the same records in triangle-id order (surface / light-pdf lookups)
 //   tri_info   2 x int4 per triangle: (material, na, nb, nc) (ta, tb, tc, mesh)
@@ -34,7 +35,9 @@ static constexpr float kEPS = 1e-3f; // cpu/defs.h:24
 //   leaf = 0x80000000 | (first_tri << 3) | (count - 1), count <= 8
 static constexpr uint32_t kLeafBit = 0x80000000u;
 static constexpr int kMaxLeaf = 8;
-static constexpr int kStackSize = 64;
+static constexpr int kStackSize = 96; // worst-case traversal stack entries (checked at BVH build)
+static constexpr int kLdsStack = 16;  // entries per lane kept in LDS
+static constexpr int32_t kEmptyChild = 0x7FFFFFFF; // unused slot of a 4-wide node
 
 struct DTex {
     int32_t image;   // -1: constant
@@ -128,7 +131,7 @@ struct DScene {
     int32_t num_lights;
     int32_t env_w, env_h;
     int32_t bg_map;
-    int32_t pad0;
+    int32_t wide; // 1: 4-wide 128-B nodes (8 float4); 0: binary 64-B nodes (4 float4)
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

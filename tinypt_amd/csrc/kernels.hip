@@ -65,11 +65,32 @@ __device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u
     return !(alpha_at(S, m.alpha, uv) < alpha_draw(ac.key, ac.depth, gid));
 }
 
+// Traversal stack: the first kLdsStack entries of every lane live in LDS,
+// entry-major ([entry][lane]), so a lane always touches its own bank whatever
+// its depth; deeper entries (rare: the host bounds the worst case by
+// kStackSize) spill to a per-lane private array.
+struct TStack {
+    int32_t *lds;
+    int stride, lane;
+    int32_t spill[kStackSize - kLdsStack];
+    __device__ __forceinline__ void set(int sp, int32_t v) {
+        if (sp < kLdsStack) lds[sp * stride + lane] = v;
+        else spill[sp - kLdsStack] = v;
+    }
+    __device__ __forceinline__ int32_t get(int sp) const {
+        return sp < kLdsStack ? lds[sp * stride + lane] : spill[sp - kLdsStack];
+    }
+};
+
+__device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V3 d, const AlphaCtx &ac, HitRec &best,
+                                           bool any, uint32_t &tris, bool count, bool &done);
+
 // Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
-// Ties on t resolve to the smaller global primitive id.
-template <bool ANY, bool COUNT>
+// Ties on t resolve to the smaller global primitive id.  WIDE: 4-wide BVH with
+// 128-B SoA nodes; otherwise the binary BVH with 64-B nodes.
+template <bool ANY, bool COUNT, bool WIDE>
 __device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
-                                        uint32_t &tris) {
+                                        uint32_t &tris, int32_t *lds) {
     HitRec best{tmax, -1, 0.f, 0.f};
     for (int i = 0; i < S.num_analytic; i++) {
         const DAnalytic &p = S.analytic[i];
@@ -93,57 +114,107 @@ __device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax,
     }
     if (S.root == -1) return best;
     V3 inv{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
-    int32_t stack[kStackSize];
+    TStack stk;
+    stk.lds = lds;
+    stk.stride = blockDim.x;
+    stk.lane = threadIdx.x;
     int sp = 0;
     int32_t cur = S.root;
     while (true) {
         if (cur >= 0) {
-            const float4 *nd = S.nodes + 4 * cur;
-            float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
             if (COUNT) nodes++;
-            float n0, n1;
-            bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, best.t, n0);
-            bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, best.t, n1);
-            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-            if (h0 && h1) {
-                int32_t nearc = (n1 < n0) ? c1 : c0;
-                int32_t farc = (n1 < n0) ? c0 : c1;
-                stack[sp++] = farc;
-                cur = nearc;
-                continue;
-            }
-            if (h0) {
-                cur = c0;
-                continue;
-            }
-            if (h1) {
-                cur = c1;
-                continue;
-            }
-        } else {
-            uint32_t ref = (uint32_t)cur;
-            int first = (int)((ref & ~kLeafBit) >> 3);
-            int cnt = (int)(ref & 7u) + 1;
-            for (int k = 0; k < cnt; k++) {
-                const float4 *tr = S.tris_bvh + 3 * (first + k);
-                float4 r0 = tr[0], r1 = tr[1], r2 = tr[2];
-                if (COUNT) tris++;
-                float t, u, v;
-                if (!tri_test(mk4(r0), mk4(r1), mk4(r2), o, d, t, u, v)) continue;
-                int32_t gid = __float_as_int(r0.w);
-                if (kEPS <= t && (t < best.t || (t == best.t && gid < best.gid))) {
-                    if (!ANY && ac.on && (__float_as_int(r2.w) & 1)) {
-                        if (!tri_alpha_pass(S, __float_as_int(r1.w), u, v, ac, gid)) continue;
-                    }
-                    best = HitRec{t, gid, u, v};
-                    if (ANY) return best;
+            if (WIDE) {
+                const float4 *nd = S.nodes + 8 * cur;
+                float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+                float tn[4];
+                int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+                float t;
+                tn[0] = (ch[0] != kEmptyChild && box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, best.t, t)) ? t : kINF;
+                tn[1] = (ch[1] != kEmptyChild && box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, best.t, t)) ? t : kINF;
+                tn[2] = (ch[2] != kEmptyChild && box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, best.t, t)) ? t : kINF;
+                tn[3] = (ch[3] != kEmptyChild && box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, best.t, t)) ? t : kINF;
+                int nh = (tn[0] < kINF) + (tn[1] < kINF) + (tn[2] < kINF) + (tn[3] < kINF);
+                if (nh > 0) {
+                    // 5-comparator sorting network on (distance, child); misses sort last
+#define TPT_CSWAP(a, b)                                                                                                \
+    if (tn[b] < tn[a]) {                                                                                               \
+        float tt = tn[a];                                                                                              \
+        tn[a] = tn[b];                                                                                                 \
+        tn[b] = tt;                                                                                                    \
+        int32_t cc = ch[a];                                                                                            \
+        ch[a] = ch[b];                                                                                                 \
+        ch[b] = cc;                                                                                                    \
+    }
+                    TPT_CSWAP(0, 1)
+                    TPT_CSWAP(2, 3)
+                    TPT_CSWAP(0, 2)
+                    TPT_CSWAP(1, 3)
+                    TPT_CSWAP(1, 2)
+#undef TPT_CSWAP
+                    // push the far hits (farthest first), continue with the nearest
+                    if (nh > 3) stk.set(sp++, ch[3]);
+                    if (nh > 2) stk.set(sp++, ch[2]);
+                    if (nh > 1) stk.set(sp++, ch[1]);
+                    cur = ch[0];
+                    continue;
+                }
+            } else {
+                const float4 *nd = S.nodes + 4 * cur;
+                float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+                float n0, n1;
+                bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, best.t, n0);
+                bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, best.t, n1);
+                int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+                if (h0 && h1) {
+                    int32_t nearc = (n1 < n0) ? c1 : c0;
+                    int32_t farc = (n1 < n0) ? c0 : c1;
+                    stk.set(sp++, farc);
+                    cur = nearc;
+                    continue;
+                }
+                if (h0) {
+                    cur = c0;
+                    continue;
+                }
+                if (h1) {
+                    cur = c1;
+                    continue;
                 }
             }
+        } else {
+            bool done = false;
+            leaf_tests(S, cur, o, d, ac, best, ANY, tris, COUNT, done);
+            if (ANY && done) return best;
         }
         if (sp == 0) break;
-        cur = stack[--sp];
+        cur = stk.get(--sp);
     }
     return best;
+}
+
+__device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V3 d, const AlphaCtx &ac, HitRec &best,
+                                           bool any, uint32_t &tris, bool count, bool &done) {
+    uint32_t ref = (uint32_t)cur;
+    int first = (int)((ref & ~kLeafBit) >> 3);
+    int cnt = (int)(ref & 7u) + 1;
+    for (int k = 0; k < cnt; k++) {
+        const float4 *tr = S.tris_bvh + 3 * (first + k);
+        float4 r0 = tr[0], r1 = tr[1], r2 = tr[2];
+        if (count) tris++;
+        float t, u, v;
+        if (!tri_test(mk4(r0), mk4(r1), mk4(r2), o, d, t, u, v)) continue;
+        int32_t gid = __float_as_int(r0.w);
+        if (kEPS <= t && (t < best.t || (t == best.t && gid < best.gid))) {
+            if (!any && ac.on && (__float_as_int(r2.w) & 1)) {
+                if (!tri_alpha_pass(S, __float_as_int(r1.w), u, v, ac, gid)) continue;
+            }
+            best = HitRec{t, gid, u, v};
+            if (any) {
+                done = true;
+                return;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- wave-level compaction helpers
@@ -208,7 +279,9 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
 }
 
 // ---------------------------------------------------------------- extend
-template <bool COUNT>
+extern __shared__ int32_t tpt_lds_stack[]; // kLdsStack * blockDim.x entries
+
+template <bool COUNT, bool WIDE>
 __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
     uint32_t n = P.counters[qsel];
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,7 +295,7 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
         int slot = queue[i];
         float4 o4 = P.org[slot], d4 = P.dir[slot];
         AlphaCtx ac{P.key[slot], P.depth[slot], true};
-        HitRec h = trace<false, COUNT>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris);
+        HitRec h = trace<false, COUNT, WIDE>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris, tpt_lds_stack);
         P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
     }
     if (COUNT) {
@@ -406,7 +479,7 @@ __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int
 }
 
 // ---------------------------------------------------------------- shadow
-template <bool COUNT>
+template <bool COUNT, bool WIDE>
 __global__ void k_shadow(DScene S, DPaths P) {
     uint32_t n = P.counters[2];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
@@ -415,7 +488,7 @@ __global__ void k_shadow(DScene S, DPaths P) {
     AlphaCtx ac{0, 0, false};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o4 = P.shadow_org[i], d4 = P.shadow_dir[i];
-        HitRec h = trace<true, false>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris);
+        HitRec h = trace<true, false, WIDE>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris, tpt_lds_stack);
         if (h.gid < 0) {
             int slot = P.shadow_slot[i];
             float4 c = P.color[slot], v = P.shadow_val[i];
@@ -489,7 +562,7 @@ __global__ void k_finalize(DRender R, DPaths P, float *out) {
 }
 
 // ---------------------------------------------------------------- ray queries (parity / bench)
-template <bool COUNT>
+template <bool COUNT, bool WIDE>
 __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_t *prim, float *tout,
                                 uint32_t *counters) {
     uint32_t nodes = 0, tris = 0;
@@ -497,7 +570,7 @@ __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<false, COUNT>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        HitRec h = trace<false, COUNT, WIDE>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, tpt_lds_stack);
         bool hit = h.gid >= 0;
         prim[i] = hit ? h.gid : -1;
         tout[i] = hit ? h.t : kINF;
@@ -508,18 +581,21 @@ __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_
     }
 }
 
+template <bool WIDE>
 __global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8_t *occ) {
     uint32_t nodes = 0, tris = 0;
     AlphaCtx ac{0, 0, false};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<true, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        HitRec h = trace<true, false, WIDE>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, tpt_lds_stack);
         occ[i] = h.gid >= 0 ? 1 : 0;
     }
 }
 
 // ---------------------------------------------------------------- host-visible launchers
+static inline size_t lds_bytes(int block) { return (size_t)kLdsStack * (size_t)block * sizeof(int32_t); }
+
 void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
                      uint64_t base_mixed, int block, hipStream_t st) {
     int grid = (n_paths + block - 1) / block;
@@ -528,20 +604,36 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
 }
 void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int qsel, int nsel, int grid, int block,
                    bool count, hipStream_t st) {
-    if (count)
-        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
-    else
-        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
+    size_t sh = lds_bytes(block);
+    if (S.wide) {
+        if (count)
+            hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+        else
+            hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+        else
+            hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+    }
 }
 void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int32_t *queue, int32_t *next, int qsel,
                   int nsel, int grid, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel);
 }
 void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
-    if (count)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(block), 0, st, S, P);
-    else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(block), 0, st, S, P);
+    size_t sh = lds_bytes(block);
+    if (S.wide) {
+        if (count)
+            hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(block), sh, st, S, P);
+        else
+            hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(block), sh, st, S, P);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(block), sh, st, S, P);
+        else
+            hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(block), sh, st, S, P);
+    }
 }
 void launch_accumulate(const DRender &R, const DPaths &P, int wave_base, int k_active, int first, hipStream_t st) {
     int grid = (R.pixels + 255) / 256;
@@ -553,14 +645,26 @@ void launch_finalize(const DRender &R, const DPaths &P, float *out, hipStream_t 
 }
 void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32_t *prim, float *t,
                           uint32_t *counters, bool count, int grid, int block, hipStream_t st) {
-    if (count)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid), dim3(block), 0, st, S, rays, n, prim, t, counters);
-    else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(block), 0, st, S, rays, n, prim, t, counters);
+    size_t sh = lds_bytes(block);
+    if (S.wide) {
+        if (count)
+            hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
+        else
+            hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
+        else
+            hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
+    }
 }
 void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint8_t *occ, int grid, int block,
                            hipStream_t st) {
-    hipLaunchKernelGGL(k_trace_occluded, dim3(grid), dim3(block), 0, st, S, rays, n, occ);
+    size_t sh = lds_bytes(block);
+    if (S.wide)
+        hipLaunchKernelGGL((k_trace_occluded<true>), dim3(grid), dim3(block), sh, st, S, rays, n, occ);
+    else
+        hipLaunchKernelGGL((k_trace_occluded<false>), dim3(grid), dim3(block), sh, st, S, rays, n, occ);
 }
 
 } // namespace tpt

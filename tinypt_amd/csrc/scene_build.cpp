@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -185,6 +186,9 @@ struct Builder {
 } // namespace
 
 void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
+    // node layout: 4-wide by default; TPT_BVH_WIDTH=2 selects the binary 64-B nodes
+    const char *width = std::getenv("TPT_BVH_WIDTH");
+    H.wide = (width && std::string(width) == "2") ? 0 : 1;
     if (d.camera.width <= 0 || d.camera.height <= 0) fail(TPT_ERR_INVALID_ARGUMENT, "camera resolution must be positive");
     if (d.num_objects < 0 || d.num_materials < 0 || d.num_images < 0 || d.num_meshes < 0 || d.num_delta_lights < 0)
         fail(TPT_ERR_INVALID_ARGUMENT, "negative table size");
@@ -346,7 +350,6 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
     if (ntri > 0) {
         Builder B(btris);
         B.build(0, ntri, 0);
-        if (B.max_depth >= kStackSize - 1) fail(TPT_ERR_RUNTIME, "BVH deeper than the traversal stack");
         H.max_depth_seen = B.max_depth;
         // flatten: inner nodes get consecutive indices (DFS pre-order), leaves become child refs
         std::vector<int> inner_index(B.nodes.size(), -1);
@@ -387,29 +390,116 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
             stk2.push_back(bn.right);
             stk2.push_back(bn.left);
         }
-        auto ref = [&](int n) -> int32_t {
+        auto leaf_ref = [&](int n) -> int32_t {
             const BNode &bn = B.nodes[n];
-            if (bn.left >= 0) return inner_index[n];
             if (bn.count < 1 || bn.count > kMaxLeaf) fail(TPT_ERR_RUNTIME, "leaf size out of range");
             return (int32_t)(kLeafBit | ((uint32_t)leaf_first[n] << 3) | (uint32_t)(bn.count - 1));
         };
-        H.nodes.resize(order.size() * 4);
-        for (size_t k = 0; k < order.size(); k++) {
-            const BNode &bn = B.nodes[order[k]];
-            const BNode &L = B.nodes[bn.left], &R = B.nodes[bn.right];
-            H.nodes[4 * k + 0] = make_float4(L.lo[0], L.lo[1], L.lo[2], L.hi[0]);
-            H.nodes[4 * k + 1] = make_float4(L.hi[1], L.hi[2], R.lo[0], R.lo[1]);
-            H.nodes[4 * k + 2] = make_float4(R.lo[2], R.hi[0], R.hi[1], R.hi[2]);
-            H.nodes[4 * k + 3] = make_float4(__int_as_float_host(ref(bn.left)), __int_as_float_host(ref(bn.right)), 0.f, 0.f);
+        if (!H.wide) {
+            auto ref = [&](int n) -> int32_t { return B.nodes[n].left >= 0 ? inner_index[n] : leaf_ref(n); };
+            H.nodes.resize(order.size() * 4);
+            for (size_t k = 0; k < order.size(); k++) {
+                const BNode &bn = B.nodes[order[k]];
+                const BNode &L = B.nodes[bn.left], &R = B.nodes[bn.right];
+                H.nodes[4 * k + 0] = make_float4(L.lo[0], L.lo[1], L.lo[2], L.hi[0]);
+                H.nodes[4 * k + 1] = make_float4(L.hi[1], L.hi[2], R.lo[0], R.lo[1]);
+                H.nodes[4 * k + 2] = make_float4(R.lo[2], R.hi[0], R.hi[1], R.hi[2]);
+                H.nodes[4 * k + 3] = make_float4(__int_as_float_host(ref(bn.left)), __int_as_float_host(ref(bn.right)), 0.f, 0.f);
+            }
+            H.root = ref(0);
+            H.max_stack = B.max_depth;
+        } else {
+            // collapse the binary tree into 4-wide nodes: open the inner child of largest
+            // surface area until four children are gathered (or only leaves remain)
+            struct W {
+                int bnode;      // binary node this wide node replaces
+                int children[4];
+                int n;
+            };
+            std::vector<W> wide;
+            std::vector<int> wide_of(B.nodes.size(), -1);
+            std::vector<int> todo{0};
+            while (!todo.empty()) {
+                int bn = todo.back();
+                todo.pop_back();
+                W w{bn, {-1, -1, -1, -1}, 0};
+                std::vector<int> ch{B.nodes[bn].left, B.nodes[bn].right};
+                while ((int)ch.size() < 4) {
+                    int best = -1;
+                    float best_area = -1.f;
+                    for (int i = 0; i < (int)ch.size(); i++) {
+                        const BNode &c = B.nodes[ch[i]];
+                        if (c.left < 0) continue;
+                        float a = Builder::area(c.lo, c.hi);
+                        if (a > best_area) {
+                            best_area = a;
+                            best = i;
+                        }
+                    }
+                    if (best < 0) break;
+                    int open = ch[best];
+                    ch.erase(ch.begin() + best);
+                    ch.insert(ch.begin() + best, B.nodes[open].right);
+                    ch.insert(ch.begin() + best, B.nodes[open].left);
+                }
+                w.n = (int)ch.size();
+                for (int i = 0; i < w.n; i++) w.children[i] = ch[i];
+                wide_of[bn] = (int)wide.size();
+                wide.push_back(w);
+                for (int i = w.n - 1; i >= 0; i--)
+                    if (B.nodes[ch[i]].left >= 0) todo.push_back(ch[i]);
+            }
+            auto ref = [&](int n) -> int32_t { return B.nodes[n].left >= 0 ? wide_of[n] : leaf_ref(n); };
+            H.nodes.resize(wide.size() * 8);
+            for (size_t k = 0; k < wide.size(); k++) {
+                const W &w = wide[k];
+                float lo[3][4], hi[3][4];
+                int32_t r[4];
+                for (int i = 0; i < 4; i++) {
+                    if (i < w.n) {
+                        const BNode &c = B.nodes[w.children[i]];
+                        for (int a = 0; a < 3; a++) {
+                            lo[a][i] = c.lo[a];
+                            hi[a][i] = c.hi[a];
+                        }
+                        r[i] = ref(w.children[i]);
+                    } else {
+                        for (int a = 0; a < 3; a++) lo[a][i] = hi[a][i] = 0.f;
+                        r[i] = kEmptyChild;
+                    }
+                }
+                for (int a = 0; a < 3; a++) {
+                    H.nodes[8 * k + 2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+                    H.nodes[8 * k + 2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+                }
+                H.nodes[8 * k + 6] = make_float4(__int_as_float_host(r[0]), __int_as_float_host(r[1]),
+                                                 __int_as_float_host(r[2]), __int_as_float_host(r[3]));
+                H.nodes[8 * k + 7] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            H.root = ref(0);
+            // worst-case stack: a node pushes (hit children - 1); bound = max over root-to-node
+            // paths of the sum of (children - 1)
+            int worst = 0;
+            std::vector<std::pair<int, int>> st{{0, 0}};
+            while (!st.empty()) {
+                auto [wi, depth] = st.back();
+                st.pop_back();
+                const W &w = wide[wi];
+                int here = depth + (w.n - 1);
+                worst = std::max(worst, here);
+                for (int i = 0; i < w.n; i++)
+                    if (B.nodes[w.children[i]].left >= 0) st.push_back({wide_of[w.children[i]], here});
+            }
+            H.max_stack = worst;
         }
-        H.root = ref(0);
+        if (H.max_stack >= kStackSize) fail(TPT_ERR_RUNTIME, "BVH worst-case stack exceeds the traversal stack");
     }
     if (H.tris.empty()) {
         H.tris.resize(3, make_float4(0, 0, 0, 0));
         H.tri_info.resize(2, make_int4(0, -1, -1, -1));
     }
     if (H.tris_bvh.empty()) H.tris_bvh.resize(3, make_float4(0, 0, 0, 0));
-    if (H.nodes.empty()) H.nodes.resize(4, make_float4(0, 0, 0, 0));
+    if (H.nodes.empty()) H.nodes.resize(8, make_float4(0, 0, 0, 0));
 
     // ---- object lights: reference DFS order (cpu/scene.cpp:9-19)
     {

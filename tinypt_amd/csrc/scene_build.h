@@ -36,7 +36,9 @@ struct HostScene {
     DTex bg{};
     float bg_rot[9];
     DCamera cam{};
-    int max_depth_seen = 0; // BVH depth (stack bound check)
+    int max_depth_seen = 0; // binary BVH depth
+    int max_stack = 0;      // worst-case traversal stack entries of the flattened tree
+    int wide = 1;           // 1: 4-wide nodes; 0: binary nodes
 };
 
 // Throws tpt_error on invalid input.
