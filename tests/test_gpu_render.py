@@ -88,6 +88,33 @@ def test_fp64_accumulation_matches(hip):
     check(g, o, o2, 0.90)
 
 
+@pytest.mark.parametrize("variant", ["reference", "importance_bounded_rr", "thin_lens"])
+def test_feature_zoo(hip, variant):
+    """Textures (f32 + RGBA8), alpha cut-out, bump, vertex normals / uvs, every BxDF kind,
+    sphere / rect / circle / mesh lights, point + distant lights, env map, thin lens."""
+    import zoo
+
+    kw = dict(max_depth=6)
+    if variant == "reference":
+        scene = zoo.feature_zoo()
+    elif variant == "importance_bounded_rr":
+        scene = zoo.feature_zoo(importance=True)
+        kw.update(max_depth=10, rr_depth=3, shadow_bounded=True)
+    else:
+        scene = zoo.feature_zoo(aperture=0.3)
+    g, o, o2 = render_both(hip, scene, 16, seed=11, hdr=True, **kw)
+    check(g, o, o2, 0.75)
+
+
+def test_feature_zoo_per_path(hip):
+    # 1 spp: every pixel is one path through the zoo; >= 95% of paths agree to 1e-4
+    import zoo
+
+    g, o, o2 = render_both(hip, zoo.feature_zoo(160, 120, importance=True), 1, seed=13, hdr=True, max_depth=8,
+                           rr_depth=4, shadow_bounded=True)
+    check(g, o, o2, 0.95)
+
+
 def test_render_is_deterministic(hip):
     from tinypt_amd import scenes
 

@@ -1,0 +1,126 @@
+"""A small scene that exercises every feature of the hot path at once (test data):
+RGB f32 and RGBA8 textures, alpha cut-out (constant and RGBA8 channel), bump map,
+vertex normals + texcoords, Lambertian / Metal / Dielectric / Glossy Blinn-Phong /
+Glossy GGX / AddBXDF / MixBXDF, sphere / rectangle / circle / mesh lights, point
+lights (with and without radius), a distant light with a cone, an environment map
+and a thin-lens camera."""
+import math
+
+import numpy as np
+
+import tinypt_amd as T
+from tinypt_amd import scenes
+
+
+def rgb(v):
+    return T.RGBTexture([float(x) for x in v])
+
+
+def mat(surface, emission=None, alpha=None, bump=None):
+    m = T.Material(surface)
+    if emission is not None:
+        m.emission_texture = rgb(emission)
+    if alpha is not None:
+        m.alpha_texture = alpha
+    if bump is not None:
+        m.bump_texture = bump
+    return m
+
+
+def checker(w, h, c0, c1, n=8):
+    y, x = np.mgrid[0:h, 0:w]
+    m = ((x * n // w + y * n // h) % 2).astype(np.float32)[..., None]
+    return (np.array(c0, np.float32) * (1 - m) + np.array(c1, np.float32) * m).astype(np.float32)
+
+
+def feature_zoo(width=96, height=72, env=True, importance=False, aperture=0.0):
+    rng = np.random.default_rng(7)
+    objs = []
+    # walls: rectangles, one with an f32 checker texture
+    tex_f32 = T.RGBTexture(T.Image.from_linear(checker(32, 32, (0.8, 0.2, 0.2), (0.2, 0.2, 0.8))))
+    floor = T.Rectangle()
+    floor.location = [0.0, 0.0, 0.0]
+    floor.dimension = [20.0, 20.0]
+    floor.material = mat(T.Lambertian(tex_f32))
+    objs.append(floor)
+    back = T.Rectangle()
+    back.location = [0.0, 6.0, 5.0]
+    back.rotation = T.rotation_from_euler([0, 1, 2], [90.0, 0.0, 0.0], True)
+    back.dimension = [20.0, 10.0]
+    back.material = mat(T.Lambertian(rgb((0.7, 0.7, 0.7))))
+    objs.append(back)
+    # spheres with the BxDF family
+    def sphere(c, r, surface, **kw):
+        s = T.Sphere()
+        s.location = [float(v) for v in c]
+        s.radius = float(r)
+        s.material = mat(surface, **kw)
+        return s
+
+    objs.append(sphere((-4, 2, 1.2), 1.2, T.Glossy(50.0, rgb((0.9, 0.8, 0.5)))))
+    objs.append(sphere((-1.5, 3, 1.0), 1.0, T.AddBXDF(T.Lambertian(rgb((0.3, 0.6, 0.3))),
+                                                       T.Glossy(200.0, rgb((0.8, 0.8, 0.8)), T.Glossy.MICRO_TROWBRIDGE_REITZ))))
+    objs.append(sphere((1.2, 2.5, 1.0), 1.0, T.MixBXDF(T.Lambertian(rgb((0.6, 0.3, 0.3))), T.Metal(rgb((0.9, 0.9, 0.9))), 0.3)))
+    objs.append(sphere((3.6, 1.5, 1.0), 1.0, T.Dielectric(rgb((0.99, 0.99, 0.99)), 1.5)))
+    objs.append(sphere((0.0, 0.5, 0.6), 0.6, T.Lambertian(rgb((0.9, 0.9, 0.2))), alpha=T.AlphaTexture(0.5)))
+    # textured mesh: icosphere with smooth normals + spherical texcoords, RGBA8 albedo,
+    # alpha cut-out from the texture's alpha channel, and a bump map
+    v, f, nrm = scenes.icosphere(2)
+    uv = np.stack([0.5 + np.arctan2(v[:, 1], v[:, 0]) / (2 * np.pi), 0.5 + np.arcsin(np.clip(v[:, 2], -1, 1)) / np.pi], 1)
+    rgba = np.zeros((64, 64, 4), np.uint8)
+    rgba[..., :3] = (checker(64, 64, (0.9, 0.5, 0.1), (0.1, 0.5, 0.9)) * 255).astype(np.uint8)
+    rgba[..., 3] = np.where((np.mgrid[0:64, 0:64][0] // 8) % 3 == 0, 0, 255).astype(np.uint8)
+    img8 = T.Image.from_linear(rgba)
+    height_map = T.Image.from_linear(rng.random((32, 32)).astype(np.float32))
+    m_tex = mat(T.Lambertian(T.RGBTexture(img8)), alpha=T.AlphaTexture(img8, 3),
+                bump=T.BumpTexture(height_map, 4.0))
+    m_plain = mat(T.Lambertian(rgb((0.5, 0.5, 0.5))), bump=T.BumpTexture(height_map, 2.0))
+    V = (v * 1.3 + np.array([2.5, 4.0, 1.5])).astype(np.float32)
+    faces = f.astype(np.int32)
+    fm = (np.arange(len(faces)) % 2).astype(np.int32)
+    mesh = T.TriangleMesh.from_arrays(V, faces, [m_tex, m_plain], fm, nrm.astype(np.float32), uv.astype(np.float32))
+    objs.append(mesh)
+    # a second mesh: an emissive triangle pair (mesh light) with a rotation
+    quad_v = np.array([[-0.5, 0, 0], [0.5, 0, 0], [0.5, 0, 1], [-0.5, 0, 1]], np.float32)
+    quad = T.TriangleMesh.from_arrays(quad_v, np.array([[0, 1, 2], [0, 2, 3]], np.int32),
+                                      [mat(T.Lambertian(rgb((0, 0, 0))), emission=(3.0, 3.0, 2.5))])
+    quad.location = [-5.0, 5.5, 3.0]
+    quad.rotation = T.rotation_from_euler([0, 1, 2], [0.0, 0.0, 20.0], True)
+    objs.append(quad)
+    # circle light
+    c = T.Circle()
+    c.location = [0.0, 3.0, 7.0]
+    c.radius = 1.5
+    c.rotation = T.rotation_from_euler([0, 1, 2], [180.0, 0.0, 0.0], True)
+    c.material = mat(T.Lambertian(rgb((0, 0, 0))), emission=(6.0, 6.0, 6.0))
+    objs.append(c)
+    # delta lights
+    p1 = T.PointLight()
+    p1.location = [-3.0, 0.0, 4.0]
+    p1.color = [1.0, 0.9, 0.8]
+    p1.power = 4.0
+    p2 = T.PointLight()
+    p2.location = [4.0, 1.0, 5.0]
+    p2.power = 3.0
+    p2.radius = 0.5
+    sun = T.DistantLight()
+    sun.rotation = T.rotation_from_direction([0.3, -0.4, 0.866])
+    sun.power = 1.5
+    sun.angle = 0.1
+    bg = None
+    if env:
+        sky = np.zeros((16, 32, 3), np.float32)
+        yy = np.linspace(0, 1, 16)[:, None]
+        sky[..., 0] = 0.3 + 0.5 * yy
+        sky[..., 1] = 0.4 + 0.4 * yy
+        sky[..., 2] = 0.6 + 0.3 * yy
+        sky[12:14, 20:23] = 8.0  # a bright patch for importance sampling
+        bg = T.EnvTexture(T.RGBTexture(T.Image.from_linear(sky)), importance)
+    cam = T.Camera()
+    cam.location = [0.0, -7.0, 3.0]
+    cam.rotation = scenes.lookat_rotation((0.0, 1.0, -0.25))
+    cam.width, cam.height = width, height
+    cam.fov = math.radians(60.0)
+    if aperture > 0:
+        cam.set_lens(aperture, 9.0)
+    return T.Scene(cam, objs, [p1, p2, sun], bg)

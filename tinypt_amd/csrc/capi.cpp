@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -28,7 +29,7 @@ void launch_accumulate(const DRender &, const DPaths &, int, int, int, hipStream
 void launch_finalize(const DRender &, const DPaths &, float *, hipStream_t);
 void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, float *, uint32_t *, bool, int, int,
                           hipStream_t);
-void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, int, int, hipStream_t);
+void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
 } // namespace tpt
 
 using namespace tpt;
@@ -65,6 +66,7 @@ struct PathBuffers {
 
 struct tpt_hip_scene {
     int device = 0;
+    int cus = 256; // compute units of the device (grid sizing)
     void *arena = nullptr;
     size_t arena_bytes = 0;
     DScene S{};
@@ -141,6 +143,15 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.env_marg = (const float *)(base + o_em);
     S.env_cond = (const float *)(base + o_ec);
     S.wide = H.wide;
+    const char *pers = std::getenv("TPT_PERSISTENT");
+    // traversal kernels: 0 = one ray per thread, 1 = persistent with per-wave refill,
+    // 2 (default) = persistent + parked leaves
+    S.persistent = 2;
+    if (pers && std::string(pers) == "0") S.persistent = 0;
+    if (pers && std::string(pers) == "1") S.persistent = 1;
+    // traversal stack: private scratch by default; TPT_LDS_STACK=1 stages it in LDS
+    const char *lds = std::getenv("TPT_LDS_STACK");
+    S.lds_stack = (lds && std::string(lds) == "1") ? 1 : 0;
     S.num_nodes = (int32_t)(H.nodes.size() / (H.wide ? 8 : 4));
     S.root = H.root;
     S.num_tris = (int32_t)(H.tris_bvh.size() / 3);
@@ -215,10 +226,10 @@ int block_size_for(int spp) { // PixelSampler ctor (cpu/path_tracer.cpp:10-21)
     return 4;
 }
 
-int grid_for(size_t n, int block) {
+int grid_for(size_t n, int block, int cus = 256) {
     size_t g = (n + block - 1) / block;
-    // grid-stride kernels: 256 CUs x 8 resident blocks of 256 threads covers the chip
-    size_t cap = (size_t)2048 * 256 / block;
+    // grid-stride / persistent kernels: 8 resident blocks of 256 threads (32 waves) per CU
+    size_t cap = (size_t)cus * 2048 / block;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     return (int)g;
@@ -250,6 +261,8 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
             throw tpt_error(TPT_ERR_HIP, "hip is not available: no HIP device visible");
         h = new tpt_hip_scene();
         HIP_OK(hipGetDevice(&h->device));
+        HIP_OK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
+        h->S.num_cu = h->cus;
         HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         upload_scene(h, H);
         HIP_OK(hipMalloc(&h->trace_counters, 64));
@@ -323,7 +336,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             int k_active = std::min(k, R.spp_eff - wave_base);
             int n = pixels * k_active;
             launch_generate(S, R, P, wave_base, n, base_mixed, block, st);
-            int grid = grid_for((size_t)n, block);
+            int grid = grid_for((size_t)n, block, h->cus);
             int cur = 0;
             for (int b = 0; b <= R.max_depth; b++) {
                 int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
@@ -409,14 +422,14 @@ static int trace_common(tpt_hip_scene *h, const float *rays, uint64_t n, void *o
             d_b = (uint8_t *)d_a + align256(bytes_a);
             HIP_OK(hipMemcpyAsync((void *)d_rays, rays, bytes_r, hipMemcpyHostToDevice, st));
         }
-        if (count) HIP_OK(hipMemsetAsync(h->trace_counters, 0, 64, st));
+        HIP_OK(hipMemsetAsync(h->trace_counters, 0, 64, st)); // traffic + persistent fetch counters
         int block = 256;
-        int grid = grid_for(n, block);
+        int grid = grid_for(n, block, h->cus);
         if (closest)
             launch_trace_closest(h->S, d_rays, n, (int32_t *)d_a, (float *)d_b, h->trace_counters, count, grid, block,
                                  st);
         else
-            launch_trace_occluded(h->S, d_rays, n, (uint8_t *)d_a, grid, block, st);
+            launch_trace_occluded(h->S, d_rays, n, (uint8_t *)d_a, h->trace_counters, grid, block, st);
         HIP_OK(hipGetLastError());
         if (!dev) {
             if (closest) {

@@ -132,6 +132,10 @@ struct DScene {
     int32_t env_w, env_h;
     int32_t bg_map;
     int32_t wide; // 1: 4-wide 128-B nodes (8 float4); 0: binary 64-B nodes (4 float4)
+    int32_t persistent; // 1: persistent traversal kernels with per-wave ray refill
+    int32_t num_cu;     // compute units of the device (host-side grid sizing)
+    int32_t lds_stack;  // 1: first kLdsStack traversal-stack entries in LDS; 0: private (scratch) stack
+    int32_t pad2;
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

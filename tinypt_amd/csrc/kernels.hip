@@ -24,16 +24,33 @@ namespace tpt {
 __device__ __forceinline__ float safe_rcp(float d) {
     return (fabsf(d) > 1e-30f) ? 1.f / d : copysignf(1e30f, d);
 }
-__device__ __forceinline__ bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, V3 o, V3 inv,
-                                        float tmax, float &tnear) {
-    float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-    float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-    float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
-    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+// Per-ray constants of the slab test: the origin and the reciprocal direction.
+// Slab distances are (l - o) * inv, exactly as the reference's AABB::intersect
+// (cpu/bvh.cpp:8-27) computes them; the far distance is widened by 2^-17
+// relative, the near distance is clamped to EPS and the culling bound is the
+// current closest t, all folded into max3 / min3.  (A fma(l, inv, -o*inv)
+// form saves 6 ops per box but needs two more live registers per lane and
+// measured 25% slower on gfx950 through spills.)
+struct RayPre {
+    V3 o, inv;
+};
+__device__ __forceinline__ RayPre ray_pre(V3 o, V3 d) {
+    RayPre r;
+    r.o = o;
+    r.inv = V3{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
+    return r;
+}
+static constexpr float kWiden = 1.f + 0x1p-17f;
+__device__ __forceinline__ float cull_bound(const RayPre &, float tmax) { return tmax; }
+__device__ __forceinline__ bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, const RayPre &r,
+                                        float tw, float &tnear) {
+    float t0x = (lx - r.o.x) * r.inv.x, t1x = (hx - r.o.x) * r.inv.x;
+    float t0y = (ly - r.o.y) * r.inv.y, t1y = (hy - r.o.y) * r.inv.y;
+    float t0z = (lz - r.o.z) * r.inv.z, t1z = (hz - r.o.z) * r.inv.z;
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), kEPS * 0.99f));
     float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-    f = f * 1.0000077f;
     tnear = n;
-    return n <= f && f >= kEPS * 0.99f && n <= tmax;
+    return n <= fminf(f * kWiden, tw);
 }
 
 struct AlphaCtx {
@@ -69,129 +86,26 @@ __device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u
 // entry-major ([entry][lane]), so a lane always touches its own bank whatever
 // its depth; deeper entries (rare: the host bounds the worst case by
 // kStackSize) spill to a per-lane private array.
+extern __shared__ int32_t tpt_lds_stack[]; // kLdsStack * blockDim.x entries
+
+// With `use_lds` off (a wave-uniform kernel argument) the whole stack is the
+// per-lane private array: scratch accesses are lane-interleaved by the
+// hardware, so they coalesce and stay in L1, with no LDS/spill branch per push.
 struct TStack {
     int32_t *lds;
     int stride, lane;
-    int32_t spill[kStackSize - kLdsStack];
+    bool use_lds;
+    int32_t spill[kStackSize];
     __device__ __forceinline__ void set(int sp, int32_t v) {
-        if (sp < kLdsStack) lds[sp * stride + lane] = v;
-        else spill[sp - kLdsStack] = v;
+        if (use_lds && sp < kLdsStack) lds[sp * stride + lane] = v;
+        else spill[sp] = v;
     }
     __device__ __forceinline__ int32_t get(int sp) const {
-        return sp < kLdsStack ? lds[sp * stride + lane] : spill[sp - kLdsStack];
+        return (use_lds && sp < kLdsStack) ? lds[sp * stride + lane] : spill[sp];
     }
 };
 
-__device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V3 d, const AlphaCtx &ac, HitRec &best,
-                                           bool any, uint32_t &tris, bool count, bool &done);
-
-// Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
-// Ties on t resolve to the smaller global primitive id.  WIDE: 4-wide BVH with
-// 128-B SoA nodes; otherwise the binary BVH with 64-B nodes.
-template <bool ANY, bool COUNT, bool WIDE>
-__device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
-                                        uint32_t &tris, int32_t *lds) {
-    HitRec best{tmax, -1, 0.f, 0.f};
-    for (int i = 0; i < S.num_analytic; i++) {
-        const DAnalytic &p = S.analytic[i];
-        float t;
-        bool into = true;
-        if (p.type == A_SPHERE) {
-            t = sphere_t(p, o, d, into);
-        } else {
-            V2 uv;
-            t = plane_t(p, o, d, kINF, uv);
-        }
-        if (t < kINF && (t < best.t || (t == best.t && p.gid < best.gid))) {
-            if (!ANY && ac.on && p.type == A_SPHERE && !S.materials[p.material].alpha_opaque) {
-                V3 n = nrm(sub(at(o, d, t), mk(p.center)));
-                V2 uv{(atan2f(n.y, n.x) + kPI) * (.5f * kINV_PI), acosf(n.z) * kINV_PI};
-                if (alpha_at(S, S.materials[p.material].alpha, uv) < alpha_draw(ac.key, ac.depth, p.gid)) continue;
-            }
-            best = HitRec{t, p.gid, into ? 1.f : 0.f, 0.f};
-            if (ANY) return best;
-        }
-    }
-    if (S.root == -1) return best;
-    V3 inv{safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)};
-    TStack stk;
-    stk.lds = lds;
-    stk.stride = blockDim.x;
-    stk.lane = threadIdx.x;
-    int sp = 0;
-    int32_t cur = S.root;
-    while (true) {
-        if (cur >= 0) {
-            if (COUNT) nodes++;
-            if (WIDE) {
-                const float4 *nd = S.nodes + 8 * cur;
-                float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
-                float tn[4];
-                int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
-                float t;
-                tn[0] = (ch[0] != kEmptyChild && box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, best.t, t)) ? t : kINF;
-                tn[1] = (ch[1] != kEmptyChild && box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, best.t, t)) ? t : kINF;
-                tn[2] = (ch[2] != kEmptyChild && box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, best.t, t)) ? t : kINF;
-                tn[3] = (ch[3] != kEmptyChild && box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, best.t, t)) ? t : kINF;
-                int nh = (tn[0] < kINF) + (tn[1] < kINF) + (tn[2] < kINF) + (tn[3] < kINF);
-                if (nh > 0) {
-                    // 5-comparator sorting network on (distance, child); misses sort last
-#define TPT_CSWAP(a, b)                                                                                                \
-    if (tn[b] < tn[a]) {                                                                                               \
-        float tt = tn[a];                                                                                              \
-        tn[a] = tn[b];                                                                                                 \
-        tn[b] = tt;                                                                                                    \
-        int32_t cc = ch[a];                                                                                            \
-        ch[a] = ch[b];                                                                                                 \
-        ch[b] = cc;                                                                                                    \
-    }
-                    TPT_CSWAP(0, 1)
-                    TPT_CSWAP(2, 3)
-                    TPT_CSWAP(0, 2)
-                    TPT_CSWAP(1, 3)
-                    TPT_CSWAP(1, 2)
-#undef TPT_CSWAP
-                    // push the far hits (farthest first), continue with the nearest
-                    if (nh > 3) stk.set(sp++, ch[3]);
-                    if (nh > 2) stk.set(sp++, ch[2]);
-                    if (nh > 1) stk.set(sp++, ch[1]);
-                    cur = ch[0];
-                    continue;
-                }
-            } else {
-                const float4 *nd = S.nodes + 4 * cur;
-                float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
-                float n0, n1;
-                bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, best.t, n0);
-                bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, best.t, n1);
-                int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-                if (h0 && h1) {
-                    int32_t nearc = (n1 < n0) ? c1 : c0;
-                    int32_t farc = (n1 < n0) ? c0 : c1;
-                    stk.set(sp++, farc);
-                    cur = nearc;
-                    continue;
-                }
-                if (h0) {
-                    cur = c0;
-                    continue;
-                }
-                if (h1) {
-                    cur = c1;
-                    continue;
-                }
-            }
-        } else {
-            bool done = false;
-            leaf_tests(S, cur, o, d, ac, best, ANY, tris, COUNT, done);
-            if (ANY && done) return best;
-        }
-        if (sp == 0) break;
-        cur = stk.get(--sp);
-    }
-    return best;
-}
-
+// Leaf: Moller-Trumbore on each triangle record (48 B) of the leaf range.
 __device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V3 d, const AlphaCtx &ac, HitRec &best,
                                            bool any, uint32_t &tris, bool count, bool &done) {
     uint32_t ref = (uint32_t)cur;
@@ -217,6 +131,135 @@ __device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V
     }
 }
 
+// Analytic primitives (the reference's linear top level, cpu/object.cpp:398-405).
+// Returns true when an any-hit query is already answered.
+template <bool ANY>
+__device__ __forceinline__ bool analytic_tests(const DScene &S, V3 o, V3 d, const AlphaCtx &ac, HitRec &best) {
+    for (int i = 0; i < S.num_analytic; i++) {
+        const DAnalytic &p = S.analytic[i];
+        float t;
+        bool into = true;
+        if (p.type == A_SPHERE) {
+            t = sphere_t(p, o, d, into);
+        } else {
+            V2 uv;
+            t = plane_t(p, o, d, kINF, uv);
+        }
+        if (t < kINF && (t < best.t || (t == best.t && p.gid < best.gid))) {
+            if (!ANY && ac.on && p.type == A_SPHERE && !S.materials[p.material].alpha_opaque) {
+                V3 n = nrm(sub(at(o, d, t), mk(p.center)));
+                V2 uv{(atan2f(n.y, n.x) + kPI) * (.5f * kINV_PI), acosf(n.z) * kINV_PI};
+                if (alpha_at(S, S.materials[p.material].alpha, uv) < alpha_draw(ac.key, ac.depth, p.gid)) continue;
+            }
+            best = HitRec{t, p.gid, into ? 1.f : 0.f, 0.f};
+            if (ANY) return true;
+        }
+    }
+    return false;
+}
+
+// One traversal step: visit node `cur` (inner node: descend to the nearest hit
+// child, push the others; leaf: test its triangles), then pop if needed.
+// Returns false once the traversal is over.  WIDE: 4-wide BVH with 128-B SoA
+// nodes; otherwise the binary BVH with 64-B nodes.
+template <bool ANY, bool COUNT, bool WIDE>
+__device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const RayPre &rp, const AlphaCtx &ac, HitRec &best,
+                                          int32_t &cur, int &sp, TStack &stk, uint32_t &nodes, uint32_t &tris) {
+    if (cur >= 0) {
+        if (COUNT) nodes++;
+        float tw = cull_bound(rp, best.t);
+        if (WIDE) {
+            const float4 *nd = S.nodes + 8 * cur;
+            float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+            float tn[4];
+            int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+            // branch-free: all four slab tests run, so the whole 128-B node is fetched
+            // up front with 7 dwordx4 loads (a short-circuit per child would turn them
+            // into conditional scalar-dword loads)
+            float t0, t1, t2, t3;
+            bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, rp, tw, t0) & (ch[0] != kEmptyChild);
+            bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, rp, tw, t1) & (ch[1] != kEmptyChild);
+            bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, rp, tw, t2) & (ch[2] != kEmptyChild);
+            bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, rp, tw, t3) & (ch[3] != kEmptyChild);
+            tn[0] = b0 ? t0 : kINF;
+            tn[1] = b1 ? t1 : kINF;
+            tn[2] = b2 ? t2 : kINF;
+            tn[3] = b3 ? t3 : kINF;
+            int nh = (tn[0] < kINF) + (tn[1] < kINF) + (tn[2] < kINF) + (tn[3] < kINF);
+            if (nh > 0) {
+                // 5-comparator sorting network on (distance, child); misses sort last
+#define TPT_CSWAP(a, b)                                                                                                \
+    if (tn[b] < tn[a]) {                                                                                               \
+        float tt = tn[a];                                                                                              \
+        tn[a] = tn[b];                                                                                                 \
+        tn[b] = tt;                                                                                                    \
+        int32_t cc = ch[a];                                                                                            \
+        ch[a] = ch[b];                                                                                                 \
+        ch[b] = cc;                                                                                                    \
+    }
+                TPT_CSWAP(0, 1)
+                TPT_CSWAP(2, 3)
+                TPT_CSWAP(0, 2)
+                TPT_CSWAP(1, 3)
+                TPT_CSWAP(1, 2)
+#undef TPT_CSWAP
+                // push the far hits (farthest first), continue with the nearest
+                if (nh > 3) stk.set(sp++, ch[3]);
+                if (nh > 2) stk.set(sp++, ch[2]);
+                if (nh > 1) stk.set(sp++, ch[1]);
+                cur = ch[0];
+                return true;
+            }
+        } else {
+            const float4 *nd = S.nodes + 4 * cur;
+            float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+            float n0, n1;
+            bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, rp, tw, n0);
+            bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, rp, tw, n1);
+            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+            if (h0 && h1) {
+                int32_t nearc = (n1 < n0) ? c1 : c0;
+                int32_t farc = (n1 < n0) ? c0 : c1;
+                stk.set(sp++, farc);
+                cur = nearc;
+                return true;
+            }
+            if (h0 || h1) {
+                cur = h0 ? c0 : c1;
+                return true;
+            }
+        }
+    } else {
+        bool done = false;
+        leaf_tests(S, cur, o, d, ac, best, ANY, tris, COUNT, done);
+        if (ANY && done) return false;
+    }
+    if (sp == 0) return false;
+    cur = stk.get(--sp);
+    return true;
+}
+
+// Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
+// Ties on t resolve to the smaller global primitive id.
+template <bool ANY, bool COUNT, bool WIDE>
+__device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
+                                        uint32_t &tris, int32_t *lds) {
+    HitRec best{tmax, -1, 0.f, 0.f};
+    if (analytic_tests<ANY>(S, o, d, ac, best)) return best;
+    if (S.root == -1) return best;
+    RayPre rp = ray_pre(o, d);
+    TStack stk;
+    stk.lds = lds;
+    stk.stride = blockDim.x;
+    stk.lane = threadIdx.x;
+    stk.use_lds = S.lds_stack != 0;
+    int sp = 0;
+    int32_t cur = S.root;
+    while (trav_step<ANY, COUNT, WIDE>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris)) {
+    }
+    return best;
+}
+
 // ---------------------------------------------------------------- wave-level compaction helpers
 __device__ __forceinline__ int wave_append(bool pred, uint32_t *counter) {
     uint64_t mask = __ballot(pred);
@@ -237,10 +280,283 @@ __device__ __forceinline__ void wave_count(uint32_t v32, uint32_t *counter) {
     if (__lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long *>(counter), v);
 }
 
+// ---------------------------------------------------------------- persistent traversal
+// Each wave keeps its 64 lanes busy: a lane that finishes its ray takes a new
+// one from a global counter as soon as kRefillLanes lanes of the wave are idle
+// (one atomic per refill for the whole wave), instead of idling until the
+// slowest ray of the wave is done.  Ray sources / sinks:
+struct SrcQueue { // extend: ray queue of slot ids -> hit records by queue position
+    const int32_t *queue;
+    const float4 *org, *dir;
+    const uint64_t *key;
+    const int32_t *depth;
+    float4 *hit;
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
+        int slot = queue[i];
+        o = mk4(org[slot]);
+        d = mk4(dir[slot]);
+        tmax = kINF;
+        ac = AlphaCtx{key[slot], depth[slot], true};
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
+        hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+    }
+};
+struct SrcShadow { // delta-light shadow rays -> add the contribution when unoccluded
+    const float4 *org, *dir, *val;
+    const int32_t *slot;
+    float4 *color;
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
+        float4 o4 = org[i];
+        o = mk4(o4);
+        d = mk4(dir[i]);
+        tmax = o4.w;
+        ac = AlphaCtx{0, 0, false};
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
+        if (h.gid < 0) {
+            int s = slot[i];
+            float4 c = color[s], v = val[i];
+            color[s] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, 0.f);
+        }
+    }
+};
+struct SrcRays { // ray-query API: (org, tmin, dir, tmax) records -> prim id + t, or occlusion byte
+    const float4 *rays;
+    int32_t *prim;
+    float *t;
+    uint8_t *occ;
+    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
+        float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        o = mk4(a);
+        d = mk4(b);
+        tmax = b.w < kINF ? b.w : kINF;
+        ac = AlphaCtx{0, 0, false};
+    }
+    __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
+        bool hit = h.gid >= 0;
+        if (occ) {
+            occ[i] = hit ? 1 : 0;
+        } else {
+            prim[i] = hit ? h.gid : -1;
+            t[i] = hit ? h.t : kINF;
+        }
+    }
+};
+
+static constexpr int kRefillLanes = 16;
+static constexpr int kLeafBatch = 16;
+static constexpr int32_t kNoNode = -1; // never a leaf reference (those stay above 0xC0000000)
+
+// Inner-node visit only: `cur` becomes the nearest hit child (inner node or
+// leaf), else the next stack entry, else kNoNode.
+template <bool COUNT, bool WIDE>
+__device__ __forceinline__ void inner_visit(const DScene &S, const RayPre &rp, float tmax, int32_t &cur, int &sp,
+                                            TStack &stk, uint32_t &nodes) {
+    if (COUNT) nodes++;
+    float tw = cull_bound(rp, tmax);
+    if (WIDE) {
+        const float4 *nd = S.nodes + 8 * cur;
+        float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+        float tn[4];
+        int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+        float t0, t1, t2, t3;
+        bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, rp, tw, t0) & (ch[0] != kEmptyChild);
+        bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, rp, tw, t1) & (ch[1] != kEmptyChild);
+        bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, rp, tw, t2) & (ch[2] != kEmptyChild);
+        bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, rp, tw, t3) & (ch[3] != kEmptyChild);
+        tn[0] = b0 ? t0 : kINF;
+        tn[1] = b1 ? t1 : kINF;
+        tn[2] = b2 ? t2 : kINF;
+        tn[3] = b3 ? t3 : kINF;
+        int nh = (int)b0 + (int)b1 + (int)b2 + (int)b3;
+        if (nh > 0) {
+#define TPT_CSWAP(a, b)                                                                                                \
+    if (tn[b] < tn[a]) {                                                                                               \
+        float tt = tn[a];                                                                                              \
+        tn[a] = tn[b];                                                                                                 \
+        tn[b] = tt;                                                                                                    \
+        int32_t cc = ch[a];                                                                                            \
+        ch[a] = ch[b];                                                                                                 \
+        ch[b] = cc;                                                                                                    \
+    }
+            TPT_CSWAP(0, 1)
+            TPT_CSWAP(2, 3)
+            TPT_CSWAP(0, 2)
+            TPT_CSWAP(1, 3)
+            TPT_CSWAP(1, 2)
+#undef TPT_CSWAP
+            if (nh > 3) stk.set(sp++, ch[3]);
+            if (nh > 2) stk.set(sp++, ch[2]);
+            if (nh > 1) stk.set(sp++, ch[1]);
+            cur = ch[0];
+            return;
+        }
+    } else {
+        const float4 *nd = S.nodes + 4 * cur;
+        float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+        float n0, n1;
+        bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, rp, tw, n0);
+        bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, rp, tw, n1);
+        int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+        if (h0 && h1) {
+            bool swap = n1 < n0;
+            stk.set(sp++, swap ? c0 : c1);
+            cur = swap ? c1 : c0;
+            return;
+        }
+        if (h0 || h1) {
+            cur = h0 ? c0 : c1;
+            return;
+        }
+    }
+    cur = sp > 0 ? stk.get(--sp) : kNoNode;
+}
+
+template <bool ANY, bool COUNT, bool WIDE, bool POSTPONE, class Src>
+__device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src, uint32_t n, uint32_t *fetch,
+                                                 uint32_t &nodes, uint32_t &tris) {
+    int32_t leaf = kNoNode; // parked leaf of this lane
+    TStack stk;
+    stk.lds = tpt_lds_stack;
+    stk.stride = blockDim.x;
+    stk.lane = threadIdx.x;
+    stk.use_lds = S.lds_stack != 0;
+    const int lane = (int)__lane_id();
+    const uint64_t lower = (1ull << lane) - 1ull;
+    int64_t ray = -1; // ray index of this lane, -1 when idle
+    bool exhausted = false;
+    V3 o{0, 0, 0}, d{0, 0, 0};
+    RayPre rp{};
+    HitRec best{kINF, -1, 0.f, 0.f};
+    AlphaCtx ac{0, 0, false};
+    int32_t cur = -1;
+    int sp = 0;
+    while (true) {
+        // refill idle lanes
+        uint64_t need = __ballot(ray < 0 && !exhausted);
+        uint64_t act = __ballot(ray >= 0);
+        int nneed = __popcll(need);
+        if (nneed > 0 && (nneed >= kRefillLanes || act == 0)) {
+            int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch, (uint32_t)nneed);
+            base = __shfl(base, leader);
+            if ((uint64_t)base + (uint64_t)nneed >= (uint64_t)n) exhausted = true;
+            if ((need >> lane) & 1ull) {
+                uint32_t idx = base + (uint32_t)__popcll(need & lower);
+                if (idx < n) {
+                    float tmax;
+                    src.load(idx, o, d, tmax, ac);
+                    best = HitRec{tmax, -1, 0.f, 0.f};
+                    if (analytic_tests<ANY>(S, o, d, ac, best) || S.root == -1) {
+                        src.store(idx, best);
+                    } else {
+                        ray = idx;
+                        rp = ray_pre(o, d);
+                        cur = S.root;
+                        sp = 0;
+                    }
+                }
+            }
+            act = __ballot(ray >= 0);
+        }
+        if (act == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        // traverse until enough lanes are idle to make a refill worthwhile
+        if (POSTPONE) {
+            // Leaves are parked (one per lane) while the lane keeps descending inner
+            // nodes; parked leaves are tested together once kLeafBatch lanes hold one
+            // or no lane has inner work left, so most wave iterations run only the
+            // inner-node code instead of inner + triangle code.
+            while (true) {
+                if (ray >= 0 && cur >= 0) inner_visit<COUNT, WIDE>(S, rp, best.t, cur, sp, stk, nodes);
+                if (ray >= 0 && leaf == kNoNode && cur != kNoNode && cur < 0) {
+                    leaf = cur;
+                    cur = sp > 0 ? stk.get(--sp) : kNoNode;
+                }
+                uint64_t lm = __ballot(ray >= 0 && leaf != kNoNode);
+                uint64_t im = __ballot(ray >= 0 && cur >= 0);
+                if (lm != 0 && (__popcll(lm) >= kLeafBatch || im == 0)) {
+                    if (ray >= 0 && leaf != kNoNode) {
+                        bool done = false;
+                        leaf_tests(S, leaf, o, d, ac, best, ANY, tris, COUNT, done);
+                        leaf = kNoNode;
+                        if (ANY && done) {
+                            cur = kNoNode;
+                            sp = 0;
+                        }
+                    }
+                }
+                if (ray >= 0 && cur == kNoNode && leaf == kNoNode) {
+                    src.store((uint32_t)ray, best);
+                    ray = -1;
+                }
+                int active = __popcll(__ballot(ray >= 0));
+                if (active == 0 || (!exhausted && 64 - active >= kRefillLanes)) break;
+            }
+        } else {
+            while (true) {
+                if (ray >= 0) {
+                    if (!trav_step<ANY, COUNT, WIDE>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris)) {
+                        src.store((uint32_t)ray, best);
+                        ray = -1;
+                    }
+                }
+                int active = __popcll(__ballot(ray >= 0));
+                if (active == 0 || (!exhausted && 64 - active >= kRefillLanes)) break;
+            }
+        }
+    }
+}
+
+template <bool COUNT, bool WIDE, bool POST>
+__global__ void k_extend_pt(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
+    uint32_t n = P.counters[qsel];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.counters[nsel_next] = 0u; // next ray queue
+        P.counters[2] = 0u;         // shadow queue
+        P.counters[13] = 0u;        // shadow fetch counter
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 8), (unsigned long long)n);
+    }
+    uint32_t nodes = 0, tris = 0;
+    SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
+    persistent_trace<false, COUNT, WIDE, POST>(S, src, n, P.counters + 12, nodes, tris);
+    if (COUNT) {
+        wave_count(nodes, P.counters + 4);
+        wave_count(tris, P.counters + 6);
+    }
+}
+
+template <bool COUNT, bool WIDE, bool POST>
+__global__ void k_shadow_pt(DScene S, DPaths P) {
+    uint32_t n = P.counters[2];
+    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 10), (unsigned long long)n);
+    uint32_t nodes = 0, tris = 0;
+    SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
+    persistent_trace<true, false, WIDE, POST>(S, src, n, P.counters + 13, nodes, tris);
+}
+
+template <bool ANY, bool COUNT, bool WIDE, bool POST>
+__global__ void k_trace_pt(DScene S, SrcRays src, uint32_t n, uint32_t *counters) {
+    uint32_t nodes = 0, tris = 0;
+    persistent_trace<ANY, COUNT, WIDE, POST>(S, src, n, counters + 8, nodes, tris);
+    if (COUNT) {
+        wave_count(nodes, counters + 0);
+        wave_count(tris, counters + 2);
+    }
+}
+
 // ---------------------------------------------------------------- generate
 __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_paths, uint64_t base_mixed) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) P.counters[0] = (uint32_t)n_paths;
+    if (j == 0) {
+        P.counters[0] = (uint32_t)n_paths;
+        P.counters[12] = 0u; // extend fetch counter
+    }
     if (j >= n_paths) return;
     int pix = j % R.pixels;
     int s = wave_base + j / R.pixels;
@@ -279,8 +595,6 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
 }
 
 // ---------------------------------------------------------------- extend
-extern __shared__ int32_t tpt_lds_stack[]; // kLdsStack * blockDim.x entries
-
 template <bool COUNT, bool WIDE>
 __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
     uint32_t n = P.counters[qsel];
@@ -377,6 +691,7 @@ __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRe
 
 __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel) {
     uint32_t n = P.counters[qsel];
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[12] = 0u; // fetch counter of the next extend
     const float LIGHT_PROB = 1.f / 5.f; // cpu/path_tracer.cpp:104
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
@@ -596,6 +911,16 @@ __global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8
 // ---------------------------------------------------------------- host-visible launchers
 static inline size_t lds_bytes(int block) { return (size_t)kLdsStack * (size_t)block * sizeof(int32_t); }
 
+// Persistent kernels launch exactly the resident grid: num_cu x (blocks per CU
+// the occupancy calculator allows for this kernel, block size and LDS).
+static int resident_grid(const DScene &S, const void *kernel, int block, int wanted) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes(block)) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    int g = S.num_cu * per_cu;
+    return wanted < g ? (wanted < 1 ? 1 : wanted) : g;
+}
+
 void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
                      uint64_t base_mixed, int block, hipStream_t st) {
     int grid = (n_paths + block - 1) / block;
@@ -605,17 +930,31 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
 void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int qsel, int nsel, int grid, int block,
                    bool count, hipStream_t st) {
     size_t sh = lds_bytes(block);
-    if (S.wide) {
-        if (count)
-            hipLaunchKernelGGL((k_extend<true, true>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
-        else
-            hipLaunchKernelGGL((k_extend<false, true>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+#define TPT_EXT(K)                                                                                                     \
+    hipLaunchKernelGGL(K, dim3(S.persistent ? resident_grid(S, (const void *)K, block, grid) : grid), dim3(block), sh,  \
+                       st, S, P, queue, qsel, nsel)
+    if (S.persistent) {
+        if (S.wide) {
+            if (S.persistent == 2) {
+                if (count) TPT_EXT((k_extend_pt<true, true, true>)); else TPT_EXT((k_extend_pt<false, true, true>));
+            } else {
+                if (count) TPT_EXT((k_extend_pt<true, true, false>)); else TPT_EXT((k_extend_pt<false, true, false>));
+            }
+        } else {
+            if (S.persistent == 2) {
+                if (count) TPT_EXT((k_extend_pt<true, false, true>)); else TPT_EXT((k_extend_pt<false, false, true>));
+            } else {
+                if (count) TPT_EXT((k_extend_pt<true, false, false>)); else TPT_EXT((k_extend_pt<false, false, false>));
+            }
+        }
     } else {
-        if (count)
-            hipLaunchKernelGGL((k_extend<true, false>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
-        else
-            hipLaunchKernelGGL((k_extend<false, false>), dim3(grid), dim3(block), sh, st, S, P, queue, qsel, nsel);
+        if (S.wide) {
+            if (count) TPT_EXT((k_extend<true, true>)); else TPT_EXT((k_extend<false, true>));
+        } else {
+            if (count) TPT_EXT((k_extend<true, false>)); else TPT_EXT((k_extend<false, false>));
+        }
     }
+#undef TPT_EXT
 }
 void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int32_t *queue, int32_t *next, int qsel,
                   int nsel, int grid, int block, hipStream_t st) {
@@ -623,17 +962,31 @@ void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int3
 }
 void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
     size_t sh = lds_bytes(block);
-    if (S.wide) {
-        if (count)
-            hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(block), sh, st, S, P);
-        else
-            hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(block), sh, st, S, P);
+#define TPT_SHD(K)                                                                                                     \
+    hipLaunchKernelGGL(K, dim3(S.persistent ? resident_grid(S, (const void *)K, block, grid) : grid), dim3(block), sh,  \
+                       st, S, P)
+    if (S.persistent) {
+        if (S.wide) {
+            if (S.persistent == 2) {
+                if (count) TPT_SHD((k_shadow_pt<true, true, true>)); else TPT_SHD((k_shadow_pt<false, true, true>));
+            } else {
+                if (count) TPT_SHD((k_shadow_pt<true, true, false>)); else TPT_SHD((k_shadow_pt<false, true, false>));
+            }
+        } else {
+            if (S.persistent == 2) {
+                if (count) TPT_SHD((k_shadow_pt<true, false, true>)); else TPT_SHD((k_shadow_pt<false, false, true>));
+            } else {
+                if (count) TPT_SHD((k_shadow_pt<true, false, false>)); else TPT_SHD((k_shadow_pt<false, false, false>));
+            }
+        }
     } else {
-        if (count)
-            hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(block), sh, st, S, P);
-        else
-            hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(block), sh, st, S, P);
+        if (S.wide) {
+            if (count) TPT_SHD((k_shadow<true, true>)); else TPT_SHD((k_shadow<false, true>));
+        } else {
+            if (count) TPT_SHD((k_shadow<true, false>)); else TPT_SHD((k_shadow<false, false>));
+        }
     }
+#undef TPT_SHD
 }
 void launch_accumulate(const DRender &R, const DPaths &P, int wave_base, int k_active, int first, hipStream_t st) {
     int grid = (R.pixels + 255) / 256;
@@ -646,6 +999,26 @@ void launch_finalize(const DRender &R, const DPaths &P, float *out, hipStream_t 
 void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32_t *prim, float *t,
                           uint32_t *counters, bool count, int grid, int block, hipStream_t st) {
     size_t sh = lds_bytes(block);
+    if (S.persistent) {
+        SrcRays src{rays, prim, t, nullptr};
+#define TPT_TR(K)                                                                                                      \
+    hipLaunchKernelGGL(K, dim3(resident_grid(S, (const void *)K, block, grid)), dim3(block), sh, st, S, src,           \
+                       (uint32_t)n, counters)
+        if (S.wide) {
+            if (S.persistent == 2) {
+                if (count) TPT_TR((k_trace_pt<false, true, true, true>)); else TPT_TR((k_trace_pt<false, false, true, true>));
+            } else {
+                if (count) TPT_TR((k_trace_pt<false, true, true, false>)); else TPT_TR((k_trace_pt<false, false, true, false>));
+            }
+        } else {
+            if (S.persistent == 2) {
+                if (count) TPT_TR((k_trace_pt<false, true, false, true>)); else TPT_TR((k_trace_pt<false, false, false, true>));
+            } else {
+                if (count) TPT_TR((k_trace_pt<false, true, false, false>)); else TPT_TR((k_trace_pt<false, false, false, false>));
+            }
+        }
+        return;
+    }
     if (S.wide) {
         if (count)
             hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
@@ -658,9 +1031,19 @@ void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32
             hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
     }
 }
-void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint8_t *occ, int grid, int block,
-                           hipStream_t st) {
+void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint8_t *occ, uint32_t *counters, int grid,
+                           int block, hipStream_t st) {
     size_t sh = lds_bytes(block);
+    if (S.persistent) {
+        SrcRays src{rays, nullptr, nullptr, occ};
+        if (S.persistent == 2) {
+            if (S.wide) TPT_TR((k_trace_pt<true, false, true, true>)); else TPT_TR((k_trace_pt<true, false, false, true>));
+        } else {
+            if (S.wide) TPT_TR((k_trace_pt<true, false, true, false>)); else TPT_TR((k_trace_pt<true, false, false, false>));
+        }
+#undef TPT_TR
+        return;
+    }
     if (S.wide)
         hipLaunchKernelGGL((k_trace_occluded<true>), dim3(grid), dim3(block), sh, st, S, rays, n, occ);
     else
