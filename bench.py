@@ -21,8 +21,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpaths/s + equal-time relMSE vs 16k-spp oracle, plain vs radiance-cache"
 HBM_PEAK_GBS = 8000.0
-# extend algorithmic bytes: per ray 32 (ray) + 4 (queue index) + 16 (hit), per node 64, per triangle 48
-RAY_BYTES, NODE_BYTES, TRI_BYTES = 52, 64, 48
+# extend algorithmic bytes: per ray 32 (ray) + 4 (queue index) + 16 (hit), per triangle record 48,
+# per node visit the node record's size (64 B for the quantized 4-wide layout; from scene_info)
+RAY_BYTES, TRI_BYTES = 52, 48
 
 
 def copy_calibration(mib=1024):
@@ -128,7 +129,7 @@ def main():
         pt.render(dev, args.spp, count_traffic=True, **kw)
         c = pt.last_stats()
         rays = c["rays_closest"]
-        alg_bytes = rays * RAY_BYTES + c["nodes_visited"] * NODE_BYTES + c["tris_tested"] * TRI_BYTES
+        alg_bytes = rays * RAY_BYTES + c["nodes_visited"] * info["node_bytes"] + c["tris_tested"] * TRI_BYTES
         ext_time_s = (sum(ext_ms) / len(ext_ms)) / 1e3
         achieved = alg_bytes / ext_time_s / 1e9
         roofline = {
@@ -139,6 +140,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "kernel": "k_extend",
+            "node_bytes": info["node_bytes"],
             "bytes_per_launch": alg_bytes / max(1, c["extend_launches"]),
             "avg_launch_ms": (sum(ext_ms) / len(ext_ms)) / max(1, c["extend_launches"]),
         }

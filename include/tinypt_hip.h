@@ -194,9 +194,11 @@ const char *tpt_hip_last_error(void);
 int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out);
 /* Frees the device arena (cuda::Scene::~Scene, cuda/scene.h:176-197). */
 void tpt_hip_scene_destroy(tpt_hip_scene *scene);
-/* Size in bytes of the device arena and of its BVH parts. */
+/* Size in bytes of the device arena and of its BVH parts; `node_bytes` is the
+ * size of one BVH node record (64 for the default quantized 4-wide layout).
+ * Any output pointer may be NULL. */
 int tpt_hip_scene_info(const tpt_hip_scene *scene, uint64_t *arena_bytes, uint64_t *num_nodes,
-                       uint64_t *num_triangles, uint64_t *num_analytic);
+                       uint64_t *num_triangles, uint64_t *num_analytic, uint64_t *node_bytes);
 
 /* PathTracer(hip).render(scene, spp): synchronous; writes H*W*3 floats (row 0
  * at the bottom) into the HOST buffer `out_rgb`.  Replaces

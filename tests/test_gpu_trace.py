@@ -2,7 +2,8 @@
 
 The oracle traverses the reference structure (linear top level, one median-split
 BVHTree(., 4) per mesh shape, recursive near-first descent; cpu/bvh.cpp:78-118);
-the HIP path traverses its own binned-SAH BVH2.  Ids must match bit-exactly and
+the HIP path traverses its own binned-SAH BVH (by default collapsed to quantized
+4-wide 64-B nodes whose child boxes are rounded outwards).  Ids must match bit-exactly and
 hit distances must be identical floats (same Moller-Trumbore op order); every
 mismatch must be a tie on t or a reference-BVH prune artefact that brute force
 (mode 1, (t, id) tie-break) decides for the HIP result.
@@ -55,6 +56,7 @@ def test_extend_1m_rays_match_oracle(room, hip):
     host, dev, oracle = room
     rays = raysets.extend_set(oracle, _room_cam(), 1_000_000, seed=0x5EED)
     print("rays generated", flush=True)
+    assert hip.scene_info(dev)["node_bytes"] == 64  # default: quantized 4-wide nodes
     gp, gt = hip.trace_closest(dev, rays)
     op, ot = oracle.trace_closest(rays, mode=0)
     hit = op >= 0
@@ -94,13 +96,16 @@ def test_occluded_matches_oracle(room, hip):
     assert len(diff) <= 10
 
 
-def test_binary_layout_matches_oracle(hip, monkeypatch):
-    # TPT_BVH_WIDTH=2 builds the binary 64-B node layout instead of the 4-wide one
+@pytest.mark.parametrize("layout,node_bytes", [("bvh2", 64), ("bvh4", 128)])
+def test_other_node_layouts_match_oracle(hip, monkeypatch, layout, node_bytes):
+    # TPT_BVH_LAYOUT selects the binary 64-B or the f32 4-wide 128-B node layout
+    # instead of the default quantized 4-wide 64-B one
     from tinypt_amd import scenes
 
-    monkeypatch.setenv("TPT_BVH_WIDTH", "2")
+    monkeypatch.setenv("TPT_BVH_LAYOUT", layout)
     host = scenes.triangle_room(64, 48)
     dev = host.to("hip")
+    assert hip.scene_info(dev)["node_bytes"] == node_bytes
     oracle = O.OracleScene(host)
     rays = raysets.extend_set(oracle, _room_cam(), 100_000, seed=21)
     gp, gt = hip.trace_closest(dev, rays)

@@ -100,7 +100,7 @@ def lib():
     L.tpt_hip_last_error.restype = ctypes.c_char_p
     L.tpt_hip_scene_create.argtypes = [vp, ctypes.POINTER(vp)]
     L.tpt_hip_scene_destroy.argtypes = [vp]
-    L.tpt_hip_scene_info.argtypes = [vp] + [ctypes.POINTER(u64)] * 4
+    L.tpt_hip_scene_info.argtypes = [vp] + [ctypes.POINTER(u64)] * 5
     L.tpt_hip_render.argtypes = [vp, vp, vp, ctypes.POINTER(_RenderStats)]
     L.tpt_hip_trace_closest.argtypes = [vp, vp, u64, vp, vp, u32, vp]
     L.tpt_hip_trace_occluded.argtypes = [vp, vp, u64, vp, u32, vp]
@@ -155,6 +155,7 @@ def last_trace_counters(scene_hip):
 
 
 def scene_info(scene_hip):
-    v = [ctypes.c_uint64() for _ in range(4)]
+    v = [ctypes.c_uint64() for _ in range(5)]
     _check(lib().tpt_hip_scene_info(scene_hip._hip_ptr(), *[ctypes.byref(x) for x in v]))
-    return dict(arena_bytes=v[0].value, nodes=v[1].value, triangles=v[2].value, analytic=v[3].value)
+    return dict(arena_bytes=v[0].value, nodes=v[1].value, triangles=v[2].value, analytic=v[3].value,
+                node_bytes=v[4].value)

@@ -53,6 +53,18 @@ int set_err(int code, const std::string &m) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Integer tuning knob from the environment, clamped to [lo, hi]; `def` when unset.
+int env_int(const char *name, int def, int lo, int hi) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return def;
+    return std::min(hi, std::max(lo, std::atoi(v)));
+}
+
+// Defaults of the persistent traversal kernels: workgroup size and the number of
+// top-of-tree nodes (64 B each) each workgroup stages in LDS.
+constexpr int kTravBlock = 512;
+constexpr int kLdsNodes = 512;
+
 struct PathBuffers {
     void *mem = nullptr;
     size_t bytes = 0;
@@ -142,7 +154,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.lights = (const DLight *)(base + o_li);
     S.env_marg = (const float *)(base + o_em);
     S.env_cond = (const float *)(base + o_ec);
-    S.wide = H.wide;
+    S.layout = H.layout;
     const char *pers = std::getenv("TPT_PERSISTENT");
     // traversal kernels: 0 = one ray per thread, 1 = persistent with per-wave refill,
     // 2 (default) = persistent + parked leaves
@@ -152,7 +164,11 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     // traversal stack: private scratch by default; TPT_LDS_STACK=1 stages it in LDS
     const char *lds = std::getenv("TPT_LDS_STACK");
     S.lds_stack = (lds && std::string(lds) == "1") ? 1 : 0;
-    S.num_nodes = (int32_t)(H.nodes.size() / (H.wide ? 8 : 4));
+    // persistent traversal workgroups and the LDS node cache of the quantized layout
+    // (the top of the tree, in BFS order at the start of the node array)
+    S.trav_block = env_int("TPT_TRAV_BLOCK", kTravBlock, 64, 1024) / 64 * 64;
+    S.lds_nodes = H.layout == kNodeBvh4Q ? std::min(H.bfs_prefix, env_int("TPT_LDS_NODES", kLdsNodes, 0, 2048)) : 0;
+    S.num_nodes = (int32_t)(H.nodes.size() / kNodeFloat4[H.layout]);
     S.root = H.root;
     S.num_tris = (int32_t)(H.tris_bvh.size() / 3);
     S.num_analytic = (int32_t)H.analytic.size();
@@ -265,6 +281,15 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         h->S.num_cu = h->cus;
         HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         upload_scene(h, H);
+        // keep the traversal kernels' dynamic LDS (node cache + stack) within one workgroup's limit
+        int max_shared = 0;
+        HIP_OK(hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+        int stack_bytes = h->S.lds_stack ? kLdsStack * h->S.trav_block * 4 : 0;
+        if (stack_bytes > max_shared) {
+            h->S.lds_stack = 0;
+            stack_bytes = 0;
+        }
+        h->S.lds_nodes = std::min(h->S.lds_nodes, (max_shared - stack_bytes) / 64);
         HIP_OK(hipMalloc(&h->trace_counters, 64));
         *out = h;
         return TPT_OK;
@@ -280,12 +305,13 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
 void tpt_hip_scene_destroy(tpt_hip_scene *scene) { delete scene; }
 
 int tpt_hip_scene_info(const tpt_hip_scene *s, uint64_t *arena_bytes, uint64_t *num_nodes, uint64_t *num_triangles,
-                       uint64_t *num_analytic) {
+                       uint64_t *num_analytic, uint64_t *node_bytes) {
     if (!s) return set_err(TPT_ERR_INVALID_ARGUMENT, "null scene");
     if (arena_bytes) *arena_bytes = s->arena_bytes;
     if (num_nodes) *num_nodes = s->num_nodes;
     if (num_triangles) *num_triangles = s->num_tris;
     if (num_analytic) *num_analytic = s->num_analytic;
+    if (node_bytes) *node_bytes = (uint64_t)kNodeFloat4[s->S.layout] * 16u;
     return TPT_OK;
 }
 

@@ -2,8 +2,13 @@
 // builder (scene_build.cpp) and the gfx950 kernels (kernels.hip).
 //
 // Arena layout (one hipMalloc per scene, every section 256-B aligned):
-//   nodes      BVH4 nodes, 128 B each: SoA child boxes lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4],
-//              child refs[4], pad  (or, when built binary, BVH2 nodes of 64 B: both child boxes + refs)
+//   nodes      one of three node layouts (DScene.layout):
+//              kNodeBvh4Q  4-wide, 64 B (default): origin.xyz, scale.xyz (powers of two), 8-bit child
+//                          bounds qlo_x[4] qhi_x[4] qlo_y[4] qhi_y[4] qlo_z[4] qhi_z[4], child refs[4];
+//                          a child box is origin + q * scale, rounded outwards at build time
+//              kNodeBvh4   4-wide, 128 B: f32 child boxes lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4],
+//                          child refs[4], pad
+//              kNodeBvh2   binary, 64 B: both child boxes (f32) + refs
 //   tris_bvh   triangles in BVH leaf order, 48 B each: (a.xyz, gid) (ab.xyz, tri_id) (ac.xyz, flags)
 //   tris       the same records in triangle-id order (surface / light-pdf lookups)
 //   tri_info   2 x int4 per triangle: (material, na, nb, nc) (ta, tb, tc, mesh)
@@ -38,6 +43,8 @@ static constexpr int kMaxLeaf = 8;
 static constexpr int kStackSize = 96; // worst-case traversal stack entries (checked at BVH build)
 static constexpr int kLdsStack = 16;  // entries per lane kept in LDS
 static constexpr int32_t kEmptyChild = 0x7FFFFFFF; // unused slot of a 4-wide node
+enum { kNodeBvh2 = 0, kNodeBvh4 = 1, kNodeBvh4Q = 2 };
+static constexpr int kNodeFloat4[3] = {4, 8, 4}; // float4 records per node, by layout
 
 struct DTex {
     int32_t image;   // -1: constant
@@ -131,11 +138,14 @@ struct DScene {
     int32_t num_lights;
     int32_t env_w, env_h;
     int32_t bg_map;
-    int32_t wide; // 1: 4-wide 128-B nodes (8 float4); 0: binary 64-B nodes (4 float4)
+    int32_t layout;     // kNodeBvh2 / kNodeBvh4 / kNodeBvh4Q
     int32_t persistent; // 1: persistent traversal kernels with per-wave ray refill
     int32_t num_cu;     // compute units of the device (host-side grid sizing)
     int32_t lds_stack;  // 1: first kLdsStack traversal-stack entries in LDS; 0: private (scratch) stack
-    int32_t pad2;
+    int32_t lds_nodes;  // quantized layout: nodes [0, lds_nodes) (the top of the tree in BFS order)
+                        // are copied into LDS by each traversal workgroup; 0 = no node cache
+    int32_t trav_block; // threads per workgroup of the persistent traversal kernels
+    int32_t pad3;
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

@@ -3,13 +3,15 @@
 // Stages per wave of paths (all state SoA in HBM, queues compacted with wave64
 // ballot + one atomic per wavefront):
 //   generate   pixel/sample -> camera ray + erand48 state        (cpu/path_tracer.cpp:77-96, camera.cpp:23-34)
-//   extend     closest hit over analytic prims + BVH2             (cpu/scene.h:27, object.cpp:389-411, bvh.cpp:78-118)
+//   extend     closest hit over analytic prims + the triangle BVH (cpu/scene.h:27, object.cpp:389-411, bvh.cpp:78-118)
 //   shade      emission, BxDF/light mixture sampling, delta-light
 //              shadow-ray emission, throughput update, RR         (cpu/path_tracer.cpp:102-168)
 //   shadow     any-hit for delta-light shadow rays                (cpu/path_tracer.cpp:129-139)
 //   accumulate per-pixel sub-pixel sums in sample order           (cpu/path_tracer.cpp:80-96)
 //   finalize   clip(sum / b^2) into the output image
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "dev_scene.h"
 
@@ -82,26 +84,42 @@ __device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u
     return !(alpha_at(S, m.alpha, uv) < alpha_draw(ac.key, ac.depth, gid));
 }
 
-// Traversal stack: the first kLdsStack entries of every lane live in LDS,
-// entry-major ([entry][lane]), so a lane always touches its own bank whatever
-// its depth; deeper entries (rare: the host bounds the worst case by
-// kStackSize) spill to a per-lane private array.
-extern __shared__ int32_t tpt_lds_stack[]; // kLdsStack * blockDim.x entries
+// Dynamic LDS of the persistent traversal kernels: [cached top nodes: 64 B each]
+// [stack: kLdsStack * blockDim.x entries when the LDS stack is on].
+extern __shared__ int32_t tpt_lds_stack[];
 
-// With `use_lds` off (a wave-uniform kernel argument) the whole stack is the
-// per-lane private array: scratch accesses are lane-interleaved by the
-// hardware, so they coalesce and stay in L1, with no LDS/spill branch per push.
+// Traversal stack: with LDS on, the first kLdsStack entries of every lane live
+// in LDS, entry-major ([entry][lane]), so a lane always touches its own bank
+// whatever its depth; deeper entries (rare: the host bounds the worst case by
+// kStackSize) spill to a per-lane private array.  With LDS off the whole stack
+// is the per-lane private array (scratch accesses are lane-interleaved by the
+// hardware, so they coalesce).
+// LDS data is addressed through address-space-3 pointers: with a generic
+// pointer the compiler merges the LDS and the private store of set() into one
+// flat store through a selected address, which pays both the vector-memory and
+// the LDS path.
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+typedef __attribute__((address_space(3))) v4f lds_f4;
+__device__ __forceinline__ float4 from_v4f(v4f v) { return make_float4(v.x, v.y, v.z, v.w); }
+template <bool LDS>
 struct TStack {
-    int32_t *lds;
+    lds_i32 *lds;
     int stride, lane;
-    bool use_lds;
     int32_t spill[kStackSize];
+    // `cached_nodes`: 64-B node records stored in front of the stack
+    __device__ __forceinline__ void init(int cached_nodes) {
+        lds = (lds_i32 *)tpt_lds_stack + 16 * cached_nodes;
+        stride = blockDim.x;
+        lane = threadIdx.x;
+    }
     __device__ __forceinline__ void set(int sp, int32_t v) {
-        if (use_lds && sp < kLdsStack) lds[sp * stride + lane] = v;
+        if (LDS && sp < kLdsStack) lds[sp * stride + lane] = v;
         else spill[sp] = v;
     }
     __device__ __forceinline__ int32_t get(int sp) const {
-        return (use_lds && sp < kLdsStack) ? lds[sp * stride + lane] : spill[sp];
+        if (LDS && sp < kLdsStack) return lds[sp * stride + lane];
+        return spill[sp];
     }
 };
 
@@ -158,36 +176,98 @@ __device__ __forceinline__ bool analytic_tests(const DScene &S, V3 o, V3 d, cons
     return false;
 }
 
-// One traversal step: visit node `cur` (inner node: descend to the nearest hit
-// child, push the others; leaf: test its triangles), then pop if needed.
-// Returns false once the traversal is over.  WIDE: 4-wide BVH with 128-B SoA
-// nodes; otherwise the binary BVH with 64-B nodes.
-template <bool ANY, bool COUNT, bool WIDE>
-__device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const RayPre &rp, const AlphaCtx &ac, HitRec &best,
-                                          int32_t &cur, int &sp, TStack &stk, uint32_t &nodes, uint32_t &tris) {
-    if (cur >= 0) {
-        if (COUNT) nodes++;
-        float tw = cull_bound(rp, best.t);
-        if (WIDE) {
-            const float4 *nd = S.nodes + 8 * cur;
-            float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
-            float tn[4];
-            int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
-            // branch-free: all four slab tests run, so the whole 128-B node is fetched
-            // up front with 7 dwordx4 loads (a short-circuit per child would turn them
-            // into conditional scalar-dword loads)
-            float t0, t1, t2, t3;
-            bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, rp, tw, t0) & (ch[0] != kEmptyChild);
-            bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, rp, tw, t1) & (ch[1] != kEmptyChild);
-            bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, rp, tw, t2) & (ch[2] != kEmptyChild);
-            bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, rp, tw, t3) & (ch[3] != kEmptyChild);
-            tn[0] = b0 ? t0 : kINF;
-            tn[1] = b1 ? t1 : kINF;
-            tn[2] = b2 ? t2 : kINF;
-            tn[3] = b3 ? t3 : kINF;
-            int nh = (tn[0] < kINF) + (tn[1] < kINF) + (tn[2] < kINF) + (tn[3] < kINF);
-            if (nh > 0) {
-                // 5-comparator sorting network on (distance, child); misses sort last
+// Slab test on precomputed slab distances (t0 / t1 per axis, any order).
+__device__ __forceinline__ bool slab_hit(float t0x, float t1x, float t0y, float t1y, float t0z, float t1z, float tw,
+                                         float &tnear) {
+    float n = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), kEPS * 0.99f));
+    float f = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    tnear = n;
+    return n <= fminf(f * kWiden, tw);
+}
+
+// Inner-node visit: tests the node's child boxes against the ray (culling at the
+// current closest t `tw`), pushes the hit children except the nearest on the
+// stack (farthest first) and returns the nearest in `nearc`; false when no child
+// is hit.  NL selects the node layout (dev_types.h).
+// `cache`: quantized nodes [0, cache) are read from the workgroup's LDS copy
+// (persistent traversal kernels), the others from the node array.
+template <int NL, class Stk>
+__device__ __forceinline__ bool visit_inner(const DScene &S, int32_t cur, const RayPre &rp, float tw, int32_t &nearc,
+                                            int &sp, Stk &stk, int cache) {
+    if (NL == kNodeBvh2) {
+        const float4 *nd = S.nodes + 4 * cur;
+        float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
+        float n0, n1;
+        bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, rp, tw, n0);
+        bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, rp, tw, n1);
+        int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
+        if (h0 && h1) {
+            bool swap = n1 < n0;
+            stk.set(sp++, swap ? c0 : c1);
+            nearc = swap ? c1 : c0;
+            return true;
+        }
+        nearc = h0 ? c0 : c1;
+        return h0 || h1;
+    }
+    float tn[4];
+    int32_t ch[4];
+    bool hb[4];
+    if (NL == kNodeBvh4) {
+        // 128-B node: f32 child boxes, 7 dwordx4 loads issued together (branch-free tests)
+        const float4 *nd = S.nodes + 8 * cur;
+        float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+        ch[0] = __float_as_int(rf.x);
+        ch[1] = __float_as_int(rf.y);
+        ch[2] = __float_as_int(rf.z);
+        ch[3] = __float_as_int(rf.w);
+        hb[0] = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, rp, tw, tn[0]);
+        hb[1] = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, rp, tw, tn[1]);
+        hb[2] = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, rp, tw, tn[2]);
+        hb[3] = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, rp, tw, tn[3]);
+    } else {
+        // 64-B quantized node (4 dwordx4 loads): bound = origin + q * scale per axis, so
+        // a slab distance is fma(q, scale * inv, (origin - o) * inv)
+        float4 n0, n1, n2, n3;
+        if (cur < cache) {
+            const lds_f4 *nd = (const lds_f4 *)tpt_lds_stack + 4 * cur;
+            n0 = from_v4f(nd[0]);
+            n1 = from_v4f(nd[1]);
+            n2 = from_v4f(nd[2]);
+            n3 = from_v4f(nd[3]);
+        } else {
+            const float4 *nd = S.nodes + 4 * cur;
+            n0 = nd[0];
+            n1 = nd[1];
+            n2 = nd[2];
+            n3 = nd[3];
+        }
+        float ax = n0.w * rp.inv.x, ay = n1.x * rp.inv.y, az = n1.y * rp.inv.z;
+        float bx = (n0.x - rp.o.x) * rp.inv.x, by = (n0.y - rp.o.y) * rp.inv.y, bz = (n0.z - rp.o.z) * rp.inv.z;
+        uint32_t qlx = __float_as_uint(n1.z), qhx = __float_as_uint(n1.w);
+        uint32_t qly = __float_as_uint(n2.x), qhy = __float_as_uint(n2.y);
+        uint32_t qlz = __float_as_uint(n2.z), qhz = __float_as_uint(n2.w);
+        ch[0] = __float_as_int(n3.x);
+        ch[1] = __float_as_int(n3.y);
+        ch[2] = __float_as_int(n3.z);
+        ch[3] = __float_as_int(n3.w);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int sh = 8 * i;
+            hb[i] = slab_hit(fmaf((float)((qlx >> sh) & 0xffu), ax, bx), fmaf((float)((qhx >> sh) & 0xffu), ax, bx),
+                             fmaf((float)((qly >> sh) & 0xffu), ay, by), fmaf((float)((qhy >> sh) & 0xffu), ay, by),
+                             fmaf((float)((qlz >> sh) & 0xffu), az, bz), fmaf((float)((qhz >> sh) & 0xffu), az, bz), tw,
+                             tn[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        hb[i] = hb[i] & (ch[i] != kEmptyChild);
+        tn[i] = hb[i] ? tn[i] : kINF;
+    }
+    int nh = (int)hb[0] + (int)hb[1] + (int)hb[2] + (int)hb[3];
+    if (nh == 0) return false;
+    // 5-comparator sorting network on (distance, child); misses sort last
 #define TPT_CSWAP(a, b)                                                                                                \
     if (tn[b] < tn[a]) {                                                                                               \
         float tt = tn[a];                                                                                              \
@@ -197,37 +277,31 @@ __device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const Ray
         ch[a] = ch[b];                                                                                                 \
         ch[b] = cc;                                                                                                    \
     }
-                TPT_CSWAP(0, 1)
-                TPT_CSWAP(2, 3)
-                TPT_CSWAP(0, 2)
-                TPT_CSWAP(1, 3)
-                TPT_CSWAP(1, 2)
+    TPT_CSWAP(0, 1)
+    TPT_CSWAP(2, 3)
+    TPT_CSWAP(0, 2)
+    TPT_CSWAP(1, 3)
+    TPT_CSWAP(1, 2)
 #undef TPT_CSWAP
-                // push the far hits (farthest first), continue with the nearest
-                if (nh > 3) stk.set(sp++, ch[3]);
-                if (nh > 2) stk.set(sp++, ch[2]);
-                if (nh > 1) stk.set(sp++, ch[1]);
-                cur = ch[0];
-                return true;
-            }
-        } else {
-            const float4 *nd = S.nodes + 4 * cur;
-            float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
-            float n0, n1;
-            bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, rp, tw, n0);
-            bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, rp, tw, n1);
-            int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-            if (h0 && h1) {
-                int32_t nearc = (n1 < n0) ? c1 : c0;
-                int32_t farc = (n1 < n0) ? c0 : c1;
-                stk.set(sp++, farc);
-                cur = nearc;
-                return true;
-            }
-            if (h0 || h1) {
-                cur = h0 ? c0 : c1;
-                return true;
-            }
+    if (nh > 3) stk.set(sp++, ch[3]);
+    if (nh > 2) stk.set(sp++, ch[2]);
+    if (nh > 1) stk.set(sp++, ch[1]);
+    nearc = ch[0];
+    return true;
+}
+
+// One traversal step: visit node `cur` (inner node: descend to the nearest hit
+// child, push the others; leaf: test its triangles), then pop if needed.
+// Returns false once the traversal is over.
+template <bool ANY, bool COUNT, int NL, class Stk>
+__device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const RayPre &rp, const AlphaCtx &ac, HitRec &best,
+                                          int32_t &cur, int &sp, Stk &stk, uint32_t &nodes, uint32_t &tris, int cache) {
+    if (cur >= 0) {
+        if (COUNT) nodes++;
+        int32_t nearc;
+        if (visit_inner<NL>(S, cur, rp, cull_bound(rp, best.t), nearc, sp, stk, cache)) {
+            cur = nearc;
+            return true;
         }
     } else {
         bool done = false;
@@ -241,21 +315,18 @@ __device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const Ray
 
 // Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
 // Ties on t resolve to the smaller global primitive id.
-template <bool ANY, bool COUNT, bool WIDE>
+template <bool ANY, bool COUNT, int NL, bool LDS>
 __device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
-                                        uint32_t &tris, int32_t *lds) {
+                                        uint32_t &tris) {
     HitRec best{tmax, -1, 0.f, 0.f};
     if (analytic_tests<ANY>(S, o, d, ac, best)) return best;
     if (S.root == -1) return best;
     RayPre rp = ray_pre(o, d);
-    TStack stk;
-    stk.lds = lds;
-    stk.stride = blockDim.x;
-    stk.lane = threadIdx.x;
-    stk.use_lds = S.lds_stack != 0;
+    TStack<LDS> stk;
+    stk.init(0);
     int sp = 0;
     int32_t cur = S.root;
-    while (trav_step<ANY, COUNT, WIDE>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris)) {
+    while (trav_step<ANY, COUNT, NL>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris, 0)) {
     }
     return best;
 }
@@ -285,6 +356,17 @@ __device__ __forceinline__ void wave_count(uint32_t v32, uint32_t *counter) {
 // one from a global counter as soon as kRefillLanes lanes of the wave are idle
 // (one atomic per refill for the whole wave), instead of idling until the
 // slowest ray of the wave is done.  Ray sources / sinks:
+// Streamed path-state records (each read or written once per launch) use
+// non-temporal loads / stores, so they do not evict BVH nodes and triangles
+// from the L2 while the traversal runs.
+__device__ __forceinline__ float4 ld_stream(const float4 *p) {
+    return from_v4f(__builtin_nontemporal_load(reinterpret_cast<const v4f *>(p)));
+}
+__device__ __forceinline__ void st_stream(float4 *p, float4 v) {
+    __builtin_nontemporal_store(v4f{v.x, v.y, v.z, v.w}, reinterpret_cast<v4f *>(p));
+}
+template <class T> __device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
+
 struct SrcQueue { // extend: ray queue of slot ids -> hit records by queue position
     const int32_t *queue;
     const float4 *org, *dir;
@@ -292,14 +374,14 @@ struct SrcQueue { // extend: ray queue of slot ids -> hit records by queue posit
     const int32_t *depth;
     float4 *hit;
     __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
-        int slot = queue[i];
-        o = mk4(org[slot]);
-        d = mk4(dir[slot]);
+        int slot = ld_stream(queue + i);
+        o = mk4(ld_stream(org + slot));
+        d = mk4(ld_stream(dir + slot));
         tmax = kINF;
-        ac = AlphaCtx{key[slot], depth[slot], true};
+        ac = AlphaCtx{ld_stream(key + slot), ld_stream(depth + slot), true};
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
-        hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+        st_stream(hit + i, make_float4(h.t, __int_as_float(h.gid), h.u, h.v));
     }
 };
 struct SrcShadow { // delta-light shadow rays -> add the contribution when unoccluded
@@ -307,9 +389,9 @@ struct SrcShadow { // delta-light shadow rays -> add the contribution when unocc
     const int32_t *slot;
     float4 *color;
     __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
-        float4 o4 = org[i];
+        float4 o4 = ld_stream(org + i);
         o = mk4(o4);
-        d = mk4(dir[i]);
+        d = mk4(ld_stream(dir + i));
         tmax = o4.w;
         ac = AlphaCtx{0, 0, false};
     }
@@ -350,78 +432,34 @@ static constexpr int32_t kNoNode = -1; // never a leaf reference (those stay abo
 
 // Inner-node visit only: `cur` becomes the nearest hit child (inner node or
 // leaf), else the next stack entry, else kNoNode.
-template <bool COUNT, bool WIDE>
+template <bool COUNT, int NL, class Stk>
 __device__ __forceinline__ void inner_visit(const DScene &S, const RayPre &rp, float tmax, int32_t &cur, int &sp,
-                                            TStack &stk, uint32_t &nodes) {
+                                            Stk &stk, uint32_t &nodes, int cache) {
     if (COUNT) nodes++;
-    float tw = cull_bound(rp, tmax);
-    if (WIDE) {
-        const float4 *nd = S.nodes + 8 * cur;
-        float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
-        float tn[4];
-        int32_t ch[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
-        float t0, t1, t2, t3;
-        bool b0 = box_hit(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, rp, tw, t0) & (ch[0] != kEmptyChild);
-        bool b1 = box_hit(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, rp, tw, t1) & (ch[1] != kEmptyChild);
-        bool b2 = box_hit(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, rp, tw, t2) & (ch[2] != kEmptyChild);
-        bool b3 = box_hit(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, rp, tw, t3) & (ch[3] != kEmptyChild);
-        tn[0] = b0 ? t0 : kINF;
-        tn[1] = b1 ? t1 : kINF;
-        tn[2] = b2 ? t2 : kINF;
-        tn[3] = b3 ? t3 : kINF;
-        int nh = (int)b0 + (int)b1 + (int)b2 + (int)b3;
-        if (nh > 0) {
-#define TPT_CSWAP(a, b)                                                                                                \
-    if (tn[b] < tn[a]) {                                                                                               \
-        float tt = tn[a];                                                                                              \
-        tn[a] = tn[b];                                                                                                 \
-        tn[b] = tt;                                                                                                    \
-        int32_t cc = ch[a];                                                                                            \
-        ch[a] = ch[b];                                                                                                 \
-        ch[b] = cc;                                                                                                    \
-    }
-            TPT_CSWAP(0, 1)
-            TPT_CSWAP(2, 3)
-            TPT_CSWAP(0, 2)
-            TPT_CSWAP(1, 3)
-            TPT_CSWAP(1, 2)
-#undef TPT_CSWAP
-            if (nh > 3) stk.set(sp++, ch[3]);
-            if (nh > 2) stk.set(sp++, ch[2]);
-            if (nh > 1) stk.set(sp++, ch[1]);
-            cur = ch[0];
-            return;
-        }
-    } else {
-        const float4 *nd = S.nodes + 4 * cur;
-        float4 a = nd[0], b = nd[1], c = nd[2], e = nd[3];
-        float n0, n1;
-        bool h0 = box_hit(a.x, a.y, a.z, a.w, b.x, b.y, rp, tw, n0);
-        bool h1 = box_hit(b.z, b.w, c.x, c.y, c.z, c.w, rp, tw, n1);
-        int32_t c0 = __float_as_int(e.x), c1 = __float_as_int(e.y);
-        if (h0 && h1) {
-            bool swap = n1 < n0;
-            stk.set(sp++, swap ? c0 : c1);
-            cur = swap ? c1 : c0;
-            return;
-        }
-        if (h0 || h1) {
-            cur = h0 ? c0 : c1;
-            return;
-        }
+    int32_t nearc;
+    if (visit_inner<NL>(S, cur, rp, cull_bound(rp, tmax), nearc, sp, stk, cache)) {
+        cur = nearc;
+        return;
     }
     cur = sp > 0 ? stk.get(--sp) : kNoNode;
 }
 
-template <bool ANY, bool COUNT, bool WIDE, bool POSTPONE, class Src>
+template <bool ANY, bool COUNT, int NL, bool POSTPONE, bool LDS, class Src>
 __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src, uint32_t n, uint32_t *fetch,
                                                  uint32_t &nodes, uint32_t &tris) {
     int32_t leaf = kNoNode; // parked leaf of this lane
-    TStack stk;
-    stk.lds = tpt_lds_stack;
-    stk.stride = blockDim.x;
-    stk.lane = threadIdx.x;
-    stk.use_lds = S.lds_stack != 0;
+    // stage the top of the tree (quantized layout) in LDS: every ray starts there
+    const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
+    if (cache > 0) {
+        lds_f4 *dst = (lds_f4 *)tpt_lds_stack;
+        for (int i = threadIdx.x; i < 4 * cache; i += blockDim.x) {
+            float4 v = S.nodes[i];
+            dst[i] = v4f{v.x, v.y, v.z, v.w};
+        }
+        __syncthreads();
+    }
+    TStack<LDS> stk;
+    stk.init(cache);
     const int lane = (int)__lane_id();
     const uint64_t lower = (1ull << lane) - 1ull;
     int64_t ray = -1; // ray index of this lane, -1 when idle
@@ -472,7 +510,7 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
             // or no lane has inner work left, so most wave iterations run only the
             // inner-node code instead of inner + triangle code.
             while (true) {
-                if (ray >= 0 && cur >= 0) inner_visit<COUNT, WIDE>(S, rp, best.t, cur, sp, stk, nodes);
+                if (ray >= 0 && cur >= 0) inner_visit<COUNT, NL>(S, rp, best.t, cur, sp, stk, nodes, cache);
                 if (ray >= 0 && leaf == kNoNode && cur != kNoNode && cur < 0) {
                     leaf = cur;
                     cur = sp > 0 ? stk.get(--sp) : kNoNode;
@@ -500,7 +538,7 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
         } else {
             while (true) {
                 if (ray >= 0) {
-                    if (!trav_step<ANY, COUNT, WIDE>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris)) {
+                    if (!trav_step<ANY, COUNT, NL>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris, cache)) {
                         src.store((uint32_t)ray, best);
                         ray = -1;
                     }
@@ -512,8 +550,14 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
     }
 }
 
-template <bool COUNT, bool WIDE, bool POST>
-__global__ void k_extend_pt(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
+// Extra attributes of the persistent traversal kernels (occupancy-tuning builds
+// define it, e.g. as __attribute__((amdgpu_waves_per_eu(8, 8)))).
+#ifndef TPT_TRAVERSAL_ATTR
+#define TPT_TRAVERSAL_ATTR
+#endif
+
+template <bool COUNT, int NL, bool POST, bool LDS>
+__global__ TPT_TRAVERSAL_ATTR void k_extend_pt(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
     uint32_t n = P.counters[qsel];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.counters[nsel_next] = 0u; // next ray queue
@@ -523,27 +567,27 @@ __global__ void k_extend_pt(DScene S, DPaths P, const int32_t *queue, int qsel, 
     }
     uint32_t nodes = 0, tris = 0;
     SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
-    persistent_trace<false, COUNT, WIDE, POST>(S, src, n, P.counters + 12, nodes, tris);
+    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, n, P.counters + 12, nodes, tris);
     if (COUNT) {
         wave_count(nodes, P.counters + 4);
         wave_count(tris, P.counters + 6);
     }
 }
 
-template <bool COUNT, bool WIDE, bool POST>
-__global__ void k_shadow_pt(DScene S, DPaths P) {
+template <bool COUNT, int NL, bool POST, bool LDS>
+__global__ TPT_TRAVERSAL_ATTR void k_shadow_pt(DScene S, DPaths P) {
     uint32_t n = P.counters[2];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 10), (unsigned long long)n);
     uint32_t nodes = 0, tris = 0;
     SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
-    persistent_trace<true, false, WIDE, POST>(S, src, n, P.counters + 13, nodes, tris);
+    persistent_trace<true, false, NL, POST, LDS>(S, src, n, P.counters + 13, nodes, tris);
 }
 
-template <bool ANY, bool COUNT, bool WIDE, bool POST>
-__global__ void k_trace_pt(DScene S, SrcRays src, uint32_t n, uint32_t *counters) {
+template <bool ANY, bool COUNT, int NL, bool POST, bool LDS>
+__global__ TPT_TRAVERSAL_ATTR void k_trace_pt(DScene S, SrcRays src, uint32_t n, uint32_t *counters) {
     uint32_t nodes = 0, tris = 0;
-    persistent_trace<ANY, COUNT, WIDE, POST>(S, src, n, counters + 8, nodes, tris);
+    persistent_trace<ANY, COUNT, NL, POST, LDS>(S, src, n, counters + 8, nodes, tris);
     if (COUNT) {
         wave_count(nodes, counters + 0);
         wave_count(tris, counters + 2);
@@ -595,7 +639,7 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
 }
 
 // ---------------------------------------------------------------- extend
-template <bool COUNT, bool WIDE>
+template <bool COUNT, int NL>
 __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
     uint32_t n = P.counters[qsel];
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -609,7 +653,7 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
         int slot = queue[i];
         float4 o4 = P.org[slot], d4 = P.dir[slot];
         AlphaCtx ac{P.key[slot], P.depth[slot], true};
-        HitRec h = trace<false, COUNT, WIDE>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris, tpt_lds_stack);
+        HitRec h = trace<false, COUNT, NL, false>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris);
         P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
     }
     if (COUNT) {
@@ -794,7 +838,7 @@ __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int
 }
 
 // ---------------------------------------------------------------- shadow
-template <bool COUNT, bool WIDE>
+template <bool COUNT, int NL>
 __global__ void k_shadow(DScene S, DPaths P) {
     uint32_t n = P.counters[2];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
@@ -803,7 +847,7 @@ __global__ void k_shadow(DScene S, DPaths P) {
     AlphaCtx ac{0, 0, false};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o4 = P.shadow_org[i], d4 = P.shadow_dir[i];
-        HitRec h = trace<true, false, WIDE>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris, tpt_lds_stack);
+        HitRec h = trace<true, false, NL, false>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris);
         if (h.gid < 0) {
             int slot = P.shadow_slot[i];
             float4 c = P.color[slot], v = P.shadow_val[i];
@@ -877,7 +921,7 @@ __global__ void k_finalize(DRender R, DPaths P, float *out) {
 }
 
 // ---------------------------------------------------------------- ray queries (parity / bench)
-template <bool COUNT, bool WIDE>
+template <bool COUNT, int NL>
 __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_t *prim, float *tout,
                                 uint32_t *counters) {
     uint32_t nodes = 0, tris = 0;
@@ -885,7 +929,7 @@ __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<false, COUNT, WIDE>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, tpt_lds_stack);
+        HitRec h = trace<false, COUNT, NL, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
         bool hit = h.gid >= 0;
         prim[i] = hit ? h.gid : -1;
         tout[i] = hit ? h.t : kINF;
@@ -896,14 +940,14 @@ __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_
     }
 }
 
-template <bool WIDE>
+template <int NL>
 __global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8_t *occ) {
     uint32_t nodes = 0, tris = 0;
     AlphaCtx ac{0, 0, false};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<true, false, WIDE>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, tpt_lds_stack);
+        HitRec h = trace<true, false, NL, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
         occ[i] = h.gid >= 0 ? 1 : 0;
     }
 }
@@ -913,9 +957,9 @@ static inline size_t lds_bytes(int block) { return (size_t)kLdsStack * (size_t)b
 
 // Persistent kernels launch exactly the resident grid: num_cu x (blocks per CU
 // the occupancy calculator allows for this kernel, block size and LDS).
-static int resident_grid(const DScene &S, const void *kernel, int block, int wanted) {
+static int resident_grid(const DScene &S, const void *kernel, int block, size_t shared, int wanted) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds_bytes(block)) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shared) != hipSuccess || per_cu < 1)
         per_cu = 1;
     int g = S.num_cu * per_cu;
     return wanted < g ? (wanted < 1 ? 1 : wanted) : g;
@@ -927,66 +971,70 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
 }
+// Compile-time dispatch on the node layout and on boolean kernel variants.
+template <int V> using IntC = std::integral_constant<int, V>;
+template <class F> static void by_layout(int nl, F &&f) {
+    if (nl == kNodeBvh2) f(IntC<kNodeBvh2>{});
+    else if (nl == kNodeBvh4) f(IntC<kNodeBvh4>{});
+    else f(IntC<kNodeBvh4Q>{});
+}
+template <class F> static void by_bool(bool b, F &&f) {
+    if (b) f(std::true_type{});
+    else f(std::false_type{});
+}
+// Persistent-kernel variants: parked leaves (S.persistent == 2) and LDS stack.
+template <class F> static void by_variant(const DScene &S, F &&f) {
+    by_bool(S.persistent == 2, [&](auto Q) { by_bool(S.lds_stack != 0, [&](auto L) { f(Q, L); }); });
+}
+// Dynamic LDS of a persistent traversal launch: cached nodes + the LDS stack.
+static size_t trav_shared(const DScene &S, int nl, bool lds_stack) {
+    size_t nodes = nl == kNodeBvh4Q ? (size_t)S.lds_nodes * 64 : 0;
+    return nodes + (lds_stack ? lds_bytes(S.trav_block) : 0);
+}
+
 void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int qsel, int nsel, int grid, int block,
                    bool count, hipStream_t st) {
-    size_t sh = lds_bytes(block);
-#define TPT_EXT(K)                                                                                                     \
-    hipLaunchKernelGGL(K, dim3(S.persistent ? resident_grid(S, (const void *)K, block, grid) : grid), dim3(block), sh,  \
-                       st, S, P, queue, qsel, nsel)
-    if (S.persistent) {
-        if (S.wide) {
-            if (S.persistent == 2) {
-                if (count) TPT_EXT((k_extend_pt<true, true, true>)); else TPT_EXT((k_extend_pt<false, true, true>));
-            } else {
-                if (count) TPT_EXT((k_extend_pt<true, true, false>)); else TPT_EXT((k_extend_pt<false, true, false>));
+    by_layout(S.layout, [&](auto L) {
+        by_bool(count, [&](auto C) {
+            constexpr int NL = decltype(L)::value;
+            constexpr bool CNT = decltype(C)::value;
+            if (!S.persistent) {
+                hipLaunchKernelGGL((k_extend<CNT, NL>), dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
+                return;
             }
-        } else {
-            if (S.persistent == 2) {
-                if (count) TPT_EXT((k_extend_pt<true, false, true>)); else TPT_EXT((k_extend_pt<false, false, true>));
-            } else {
-                if (count) TPT_EXT((k_extend_pt<true, false, false>)); else TPT_EXT((k_extend_pt<false, false, false>));
-            }
-        }
-    } else {
-        if (S.wide) {
-            if (count) TPT_EXT((k_extend<true, true>)); else TPT_EXT((k_extend<false, true>));
-        } else {
-            if (count) TPT_EXT((k_extend<true, false>)); else TPT_EXT((k_extend<false, false>));
-        }
-    }
-#undef TPT_EXT
+            by_variant(S, [&](auto Q, auto D) {
+                constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
+                const int tb = S.trav_block;
+                size_t sh = trav_shared(S, NL, LDS);
+                int g = resident_grid(S, (const void *)(k_extend_pt<CNT, NL, POST, LDS>), tb, sh, grid);
+                hipLaunchKernelGGL((k_extend_pt<CNT, NL, POST, LDS>), dim3(g), dim3(tb), sh, st, S, P, queue, qsel,
+                                   nsel);
+            });
+        });
+    });
 }
 void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int32_t *queue, int32_t *next, int qsel,
                   int nsel, int grid, int block, hipStream_t st) {
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel);
 }
 void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
-    size_t sh = lds_bytes(block);
-#define TPT_SHD(K)                                                                                                     \
-    hipLaunchKernelGGL(K, dim3(S.persistent ? resident_grid(S, (const void *)K, block, grid) : grid), dim3(block), sh,  \
-                       st, S, P)
-    if (S.persistent) {
-        if (S.wide) {
-            if (S.persistent == 2) {
-                if (count) TPT_SHD((k_shadow_pt<true, true, true>)); else TPT_SHD((k_shadow_pt<false, true, true>));
-            } else {
-                if (count) TPT_SHD((k_shadow_pt<true, true, false>)); else TPT_SHD((k_shadow_pt<false, true, false>));
+    by_layout(S.layout, [&](auto L) {
+        by_bool(count, [&](auto C) {
+            constexpr int NL = decltype(L)::value;
+            constexpr bool CNT = decltype(C)::value;
+            if (!S.persistent) {
+                hipLaunchKernelGGL((k_shadow<CNT, NL>), dim3(grid), dim3(block), 0, st, S, P);
+                return;
             }
-        } else {
-            if (S.persistent == 2) {
-                if (count) TPT_SHD((k_shadow_pt<true, false, true>)); else TPT_SHD((k_shadow_pt<false, false, true>));
-            } else {
-                if (count) TPT_SHD((k_shadow_pt<true, false, false>)); else TPT_SHD((k_shadow_pt<false, false, false>));
-            }
-        }
-    } else {
-        if (S.wide) {
-            if (count) TPT_SHD((k_shadow<true, true>)); else TPT_SHD((k_shadow<false, true>));
-        } else {
-            if (count) TPT_SHD((k_shadow<true, false>)); else TPT_SHD((k_shadow<false, false>));
-        }
-    }
-#undef TPT_SHD
+            by_variant(S, [&](auto Q, auto D) {
+                constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
+                const int tb = S.trav_block;
+                size_t sh = trav_shared(S, NL, LDS);
+                int g = resident_grid(S, (const void *)(k_shadow_pt<CNT, NL, POST, LDS>), tb, sh, grid);
+                hipLaunchKernelGGL((k_shadow_pt<CNT, NL, POST, LDS>), dim3(g), dim3(tb), sh, st, S, P);
+            });
+        });
+    });
 }
 void launch_accumulate(const DRender &R, const DPaths &P, int wave_base, int k_active, int first, hipStream_t st) {
     int grid = (R.pixels + 255) / 256;
@@ -998,56 +1046,45 @@ void launch_finalize(const DRender &R, const DPaths &P, float *out, hipStream_t 
 }
 void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32_t *prim, float *t,
                           uint32_t *counters, bool count, int grid, int block, hipStream_t st) {
-    size_t sh = lds_bytes(block);
-    if (S.persistent) {
-        SrcRays src{rays, prim, t, nullptr};
-#define TPT_TR(K)                                                                                                      \
-    hipLaunchKernelGGL(K, dim3(resident_grid(S, (const void *)K, block, grid)), dim3(block), sh, st, S, src,           \
-                       (uint32_t)n, counters)
-        if (S.wide) {
-            if (S.persistent == 2) {
-                if (count) TPT_TR((k_trace_pt<false, true, true, true>)); else TPT_TR((k_trace_pt<false, false, true, true>));
-            } else {
-                if (count) TPT_TR((k_trace_pt<false, true, true, false>)); else TPT_TR((k_trace_pt<false, false, true, false>));
+    by_layout(S.layout, [&](auto L) {
+        by_bool(count, [&](auto C) {
+            constexpr int NL = decltype(L)::value;
+            constexpr bool CNT = decltype(C)::value;
+            if (!S.persistent) {
+                hipLaunchKernelGGL((k_trace_closest<CNT, NL>), dim3(grid), dim3(block), 0, st, S, rays, n, prim, t,
+                                   counters);
+                return;
             }
-        } else {
-            if (S.persistent == 2) {
-                if (count) TPT_TR((k_trace_pt<false, true, false, true>)); else TPT_TR((k_trace_pt<false, false, false, true>));
-            } else {
-                if (count) TPT_TR((k_trace_pt<false, true, false, false>)); else TPT_TR((k_trace_pt<false, false, false, false>));
-            }
-        }
-        return;
-    }
-    if (S.wide) {
-        if (count)
-            hipLaunchKernelGGL((k_trace_closest<true, true>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
-        else
-            hipLaunchKernelGGL((k_trace_closest<false, true>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
-    } else {
-        if (count)
-            hipLaunchKernelGGL((k_trace_closest<true, false>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
-        else
-            hipLaunchKernelGGL((k_trace_closest<false, false>), dim3(grid), dim3(block), sh, st, S, rays, n, prim, t, counters);
-    }
+            SrcRays src{rays, prim, t, nullptr};
+            by_variant(S, [&](auto Q, auto D) {
+                constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
+                const int tb = S.trav_block;
+                size_t sh = trav_shared(S, NL, LDS);
+                int g = resident_grid(S, (const void *)(k_trace_pt<false, CNT, NL, POST, LDS>), tb, sh, grid);
+                hipLaunchKernelGGL((k_trace_pt<false, CNT, NL, POST, LDS>), dim3(g), dim3(tb), sh, st, S, src,
+                                   (uint32_t)n, counters);
+            });
+        });
+    });
 }
 void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint8_t *occ, uint32_t *counters, int grid,
                            int block, hipStream_t st) {
-    size_t sh = lds_bytes(block);
-    if (S.persistent) {
-        SrcRays src{rays, nullptr, nullptr, occ};
-        if (S.persistent == 2) {
-            if (S.wide) TPT_TR((k_trace_pt<true, false, true, true>)); else TPT_TR((k_trace_pt<true, false, false, true>));
-        } else {
-            if (S.wide) TPT_TR((k_trace_pt<true, false, true, false>)); else TPT_TR((k_trace_pt<true, false, false, false>));
+    by_layout(S.layout, [&](auto L) {
+        constexpr int NL = decltype(L)::value;
+        if (!S.persistent) {
+            hipLaunchKernelGGL((k_trace_occluded<NL>), dim3(grid), dim3(block), 0, st, S, rays, n, occ);
+            return;
         }
-#undef TPT_TR
-        return;
-    }
-    if (S.wide)
-        hipLaunchKernelGGL((k_trace_occluded<true>), dim3(grid), dim3(block), sh, st, S, rays, n, occ);
-    else
-        hipLaunchKernelGGL((k_trace_occluded<false>), dim3(grid), dim3(block), sh, st, S, rays, n, occ);
+        SrcRays src{rays, nullptr, nullptr, occ};
+        by_variant(S, [&](auto Q, auto D) {
+            constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
+            const int tb = S.trav_block;
+            size_t sh = trav_shared(S, NL, LDS);
+            int g = resident_grid(S, (const void *)(k_trace_pt<true, false, NL, POST, LDS>), tb, sh, grid);
+            hipLaunchKernelGGL((k_trace_pt<true, false, NL, POST, LDS>), dim3(g), dim3(tb), sh, st, S, src,
+                               (uint32_t)n, counters);
+        });
+    });
 }
 
 } // namespace tpt

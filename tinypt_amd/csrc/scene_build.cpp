@@ -4,8 +4,10 @@
 // Replaces cuda::Scene(const cpu::Scene&) (li-plus/tinypt src/tinypt/cuda/scene.h:95-174),
 // which mirrored every object with its own cudaMalloc and rebuilt one heap-indexed
 // median-split BVHTree(faces, 4) per mesh (cuda/object.h:338-374, 478-540).  Here all
-// triangles of all meshes share ONE binned-SAH BVH2 with 64-byte nodes; analytic
-// spheres / rectangles / circles stay a linear list, as the reference's top level is.
+// triangles of all meshes share ONE binned-SAH BVH, collapsed to 4-wide nodes of 64
+// bytes with 8-bit quantized child boxes (or kept binary / f32 4-wide for A/B runs);
+// analytic spheres / rectangles / circles stay a linear list, as the reference's top
+// level is.
 #include "scene_build.h"
 
 #include <algorithm>
@@ -183,12 +185,81 @@ struct Builder {
     }
 };
 
+// Quantized 4-wide node (kNodeBvh4Q, 64 B = 4 float4):
+//   [0] origin.x origin.y origin.z scale.x
+//   [1] scale.y  scale.z  qlo_x    qhi_x        (q*: four 8-bit values, byte i = child i)
+//   [2] qlo_y    qhi_y    qlo_z    qhi_z
+//   [3] child refs 0..3
+// A child's box along an axis is [origin + qlo * scale, origin + qhi * scale].
+// The scale is a power of two, so q * scale is exact, and every decoded bound lies
+// outside the f32 child bound by at least half a quantum.  The kernel computes a
+// slab distance as fma(q, scale * inv, (origin - o) * inv); its rounding error is
+// about 2^-23 |origin - o| |inv|, so the quantum is kept >= 2^-16 of the scene's
+// magnitude `scene_mag` (max |coordinate| of the scene bounds): the half-quantum
+// margin then exceeds that error 64 times over for any ray origin within the
+// scene's bounds.
+void quantize_wide_node(const float lo[3][4], const float hi[3][4], int n, const int32_t ref[4], double scene_mag,
+                        float4 out[4]) {
+    float origin[3], scale[3];
+    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    for (int a = 0; a < 3; a++) {
+        double plo = lo[a][0], phi = hi[a][0];
+        for (int i = 1; i < n; i++) {
+            plo = std::min(plo, (double)lo[a][i]);
+            phi = std::max(phi, (double)hi[a][i]);
+        }
+        double want = std::max({(phi - plo) / 250.0, scene_mag * 0x1p-16, 0x1p-100});
+        int e;
+        std::frexp(want, &e); // want in [2^(e-1), 2^e)
+        double s = std::ldexp(1.0, e);
+        for (int attempt = 0;; attempt++) {
+            if (attempt > 8) fail(TPT_ERR_RUNTIME, "BVH node quantization failed");
+            float px = (float)(plo - s);
+            if ((double)px > plo - s) px = std::nextafter(px, -INFINITY);
+            bool ok = true;
+            uint32_t ql = 0, qh = 0;
+            for (int i = 0; i < n && ok; i++) {
+                double l = std::floor(((double)lo[a][i] - (double)px) / s - 0.5);
+                double h = std::ceil(((double)hi[a][i] - (double)px) / s + 0.5);
+                if (l < 0.0 || h > 255.0 || l > h) {
+                    ok = false;
+                    break;
+                }
+                // decoded bounds are exact in double: px + q * s
+                if ((double)px + l * s > (double)lo[a][i] - 0.5 * s || (double)px + h * s < (double)hi[a][i] + 0.5 * s) {
+                    ok = false;
+                    break;
+                }
+                ql |= (uint32_t)l << (8 * i);
+                qh |= (uint32_t)h << (8 * i);
+            }
+            if (ok) {
+                origin[a] = px;
+                scale[a] = (float)s;
+                qlo[a] = ql;
+                qhi[a] = qh;
+                break;
+            }
+            s *= 2.0;
+        }
+    }
+    auto u = [](uint32_t v) { return __int_as_float_host((int32_t)v); };
+    out[0] = make_float4(origin[0], origin[1], origin[2], scale[0]);
+    out[1] = make_float4(scale[1], scale[2], u(qlo[0]), u(qhi[0]));
+    out[2] = make_float4(u(qlo[1]), u(qhi[1]), u(qlo[2]), u(qhi[2]));
+    out[3] = make_float4(__int_as_float_host(ref[0]), __int_as_float_host(ref[1]), __int_as_float_host(ref[2]),
+                         __int_as_float_host(ref[3]));
+}
+
 } // namespace
 
 void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
-    // node layout: 4-wide by default; TPT_BVH_WIDTH=2 selects the binary 64-B nodes
-    const char *width = std::getenv("TPT_BVH_WIDTH");
-    H.wide = (width && std::string(width) == "2") ? 0 : 1;
+    // node layout: quantized 4-wide 64-B nodes by default; TPT_BVH_LAYOUT=bvh4 selects
+    // the f32 4-wide 128-B nodes, TPT_BVH_LAYOUT=bvh2 the binary 64-B nodes
+    const char *lay = std::getenv("TPT_BVH_LAYOUT");
+    H.layout = kNodeBvh4Q;
+    if (lay && std::string(lay) == "bvh4") H.layout = kNodeBvh4;
+    if (lay && std::string(lay) == "bvh2") H.layout = kNodeBvh2;
     if (d.camera.width <= 0 || d.camera.height <= 0) fail(TPT_ERR_INVALID_ARGUMENT, "camera resolution must be positive");
     if (d.num_objects < 0 || d.num_materials < 0 || d.num_images < 0 || d.num_meshes < 0 || d.num_delta_lights < 0)
         fail(TPT_ERR_INVALID_ARGUMENT, "negative table size");
@@ -395,7 +466,7 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
             if (bn.count < 1 || bn.count > kMaxLeaf) fail(TPT_ERR_RUNTIME, "leaf size out of range");
             return (int32_t)(kLeafBit | ((uint32_t)leaf_first[n] << 3) | (uint32_t)(bn.count - 1));
         };
-        if (!H.wide) {
+        if (H.layout == kNodeBvh2) {
             auto ref = [&](int n) -> int32_t { return B.nodes[n].left >= 0 ? inner_index[n] : leaf_ref(n); };
             H.nodes.resize(order.size() * 4);
             for (size_t k = 0; k < order.size(); k++) {
@@ -416,6 +487,9 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
                 int children[4];
                 int n;
             };
+            double scene_mag = 0.0;
+            for (int a = 0; a < 3; a++)
+                scene_mag = std::max({scene_mag, std::fabs((double)B.nodes[0].lo[a]), std::fabs((double)B.nodes[0].hi[a])});
             std::vector<W> wide;
             std::vector<int> wide_of(B.nodes.size(), -1);
             std::vector<int> todo{0};
@@ -449,10 +523,27 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
                 for (int i = w.n - 1; i >= 0; i--)
                     if (B.nodes[ch[i]].left >= 0) todo.push_back(ch[i]);
             }
-            auto ref = [&](int n) -> int32_t { return B.nodes[n].left >= 0 ? wide_of[n] : leaf_ref(n); };
-            H.nodes.resize(wide.size() * 8);
-            for (size_t k = 0; k < wide.size(); k++) {
-                const W &w = wide[k];
+            // node order: the first kBfsPrefix nodes in breadth-first order (the top of
+            // the tree, which the traversal kernels may stage in LDS), the rest in the
+            // depth-first pre-order they were created in
+            std::vector<int> pos(wide.size(), -1);
+            int next = 0;
+            std::vector<int> bfs{0};
+            for (size_t qh = 0; qh < bfs.size() && next < kBfsPrefix; qh++) {
+                const W &w = wide[bfs[qh]];
+                pos[bfs[qh]] = next++;
+                for (int i = 0; i < w.n; i++)
+                    if (B.nodes[w.children[i]].left >= 0) bfs.push_back(wide_of[w.children[i]]);
+            }
+            H.bfs_prefix = next;
+            for (size_t k = 0; k < wide.size(); k++)
+                if (pos[k] < 0) pos[k] = next++;
+            auto ref = [&](int n) -> int32_t { return B.nodes[n].left >= 0 ? pos[wide_of[n]] : leaf_ref(n); };
+            const int nf4 = kNodeFloat4[H.layout];
+            H.nodes.resize(wide.size() * nf4);
+            for (size_t wk = 0; wk < wide.size(); wk++) {
+                const W &w = wide[wk];
+                const size_t k = (size_t)pos[wk];
                 float lo[3][4], hi[3][4];
                 int32_t r[4];
                 for (int i = 0; i < 4; i++) {
@@ -467,6 +558,10 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
                         for (int a = 0; a < 3; a++) lo[a][i] = hi[a][i] = 0.f;
                         r[i] = kEmptyChild;
                     }
+                }
+                if (H.layout == kNodeBvh4Q) {
+                    quantize_wide_node(lo, hi, w.n, r, scene_mag, &H.nodes[4 * k]);
+                    continue;
                 }
                 for (int a = 0; a < 3; a++) {
                     H.nodes[8 * k + 2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);

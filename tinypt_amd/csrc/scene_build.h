@@ -38,8 +38,12 @@ struct HostScene {
     DCamera cam{};
     int max_depth_seen = 0; // binary BVH depth
     int max_stack = 0;      // worst-case traversal stack entries of the flattened tree
-    int wide = 1;           // 1: 4-wide nodes; 0: binary nodes
+    int layout = kNodeBvh4Q; // node layout (dev_types.h)
+    int bfs_prefix = 0;      // 4-wide layouts: nodes [0, bfs_prefix) are the top of the tree in BFS order
 };
+
+// Number of 4-wide nodes laid out breadth-first at the start of the node array.
+static constexpr int kBfsPrefix = 4096;
 
 // Throws tpt_error on invalid input.
 void build_host_scene(const tpt_scene_desc &desc, HostScene &out);
