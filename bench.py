@@ -8,6 +8,14 @@ Prints ONE JSON line with Mpaths/s, the extend kernel's roofline figure
 of the extend launches) and the CPU oracle timed on a bounded sample.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--spp S] [--no-cpu]
+
+Other workloads (one JSON line each, same fields):
+  --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
+                      16-256 spp) through PathTracer.render_batch: one shared
+                      path pool, one captured pipeline wave replayed, autotuned
+                      queue capacity and workgroup size; per-scene completion
+                      times and the same scenes rendered one by one
+  --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
 """
 import argparse
 import json
@@ -81,6 +89,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--workload", choices=["room", "batch", "spheres"], default="room")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--no-seq", action="store_true", help="batch: skip the one-by-one comparison")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -97,6 +108,12 @@ def main():
     if tinypt_amd.hip_device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    if args.workload == "spheres":
+        bench_spheres(args)
+        return
     t0 = time.perf_counter()
     host = scenes.triangle_room(args.width, args.height)
     dev = host.to("hip")
@@ -174,6 +191,120 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    print(json.dumps(out))
+
+
+def base_line(value, ms_per_step, args, config, dtype="f32"):
+    return {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": config,
+    }
+
+
+def bench_batch(args):
+    """64 independent seeded rooms through one captured pipeline wave."""
+    import tinypt_amd
+    from tinypt_amd import scenes
+
+    specs = scenes.batch_specs(count=args.batch_count)
+    t0 = time.perf_counter()
+    hosts = scenes.batch_scenes(specs)
+    devs = [h.to("hip") for h in hosts]
+    print(f"# {len(devs)} scenes ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    spps = [sp["spp"] for sp in specs]
+    seeds = [sp["seed"] for sp in specs]
+    kw = dict(max_depth=8, rr_depth=5)
+    pt = tinypt_amd.PathTracer("hip")
+    t = time.perf_counter()
+    pt.render_batch(devs, spps, seeds, **kw)  # autotunes queue capacity and workgroup size
+    st = pt.last_batch_stats()
+    cap, block = st["queue_capacity"], st["block_size"]
+    print(f"# tuned in {st['tune_ms']:.0f} ms: capacity {cap}, block {block}; first batch {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    for i in range(max(0, args.warmup - 1)):
+        pt.render_batch(devs, spps, seeds, queue_capacity=cap, block_size=block, **kw)
+    ms = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        pt.render_batch(devs, spps, seeds, queue_capacity=cap, block_size=block, **kw)
+        ms.append((time.perf_counter() - t) * 1e3)
+        st = pt.last_batch_stats()
+        print(f"# step {i}: {ms[-1]:.1f} ms ({st['waves']} waves)", file=sys.stderr, flush=True)
+    ms_per_step = sum(ms) / len(ms)
+    paths = st["paths"]
+    done = sorted(st["scene_ms"])
+    out = base_line(paths / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
+        "workload": f"batch_{len(devs)}_rooms_2k-500k_tris_128-1024px_16-256spp_depth8_rr5",
+        "tracer": "plain HIP wavefront, one pool, captured pipeline wave replayed",
+        "paths_per_step": paths,
+        "queue_capacity": cap,
+        "block_size": block,
+        "waves": st["waves"],
+        "tune_ms": round(st["tune_ms"], 1),
+        "triangles": [min(sp["tris"] for sp in specs), max(sp["tris"] for sp in specs)],
+    })
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    out["scene_completion_ms"] = {"min": round(done[0], 1), "median": round(done[len(done) // 2], 1),
+                                  "p90": round(done[int(0.9 * (len(done) - 1))], 1), "max": round(done[-1], 1)}
+    if not args.no_seq:
+        t = time.perf_counter()
+        for d, n, sd in zip(devs, spps, seeds):
+            pt.render(d, n, seed=sd, **kw)
+        seq_s = time.perf_counter() - t
+        out["sequential_mpaths_per_s"] = round(paths / seq_s / 1e6, 2)
+        out["batch_speedup_vs_sequential"] = round(seq_s * 1e3 / ms_per_step, 3)
+    print(json.dumps(out))
+
+
+def bench_spheres(args):
+    """The analytic two-spheres box (9 spheres), 512x384 at 64 spp, depth 10, RR after 5."""
+    import tinypt_amd
+    from tinypt_amd import scenes
+
+    c = scenes.CONFIG0
+    dev = scenes.cornell_spheres(c["width"], c["height"]).to("hip")
+    pt = tinypt_amd.PathTracer("hip")
+    kw = dict(max_depth=c["max_depth"], rr_depth=c["rr_depth"], seed=1)
+    for _ in range(args.warmup):
+        pt.render(dev, c["spp"], **kw)
+    ms = []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        pt.render(dev, c["spp"], **kw)
+        ms.append((time.perf_counter() - t) * 1e3)
+    st = pt.last_stats()
+    ms_per_step = sum(ms) / len(ms)
+    out = base_line(st["paths"] / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
+        "workload": f"cornell_spheres_9_{c['width']}x{c['height']}_{c['spp']}spp_depth10_rr5",
+        "tracer": "plain HIP wavefront",
+        "paths_per_step": st["paths"],
+    })
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+
+        oracle = O.OracleScene(scenes.cornell_spheres(c["width"], c["height"]))
+        threads = len(os.sched_getaffinity(0))
+        p = O.params(c["spp"], max_depth=c["max_depth"], rr_depth=c["rr_depth"], seed=1)
+        t = time.perf_counter()
+        oracle.render(c["height"], p, 0, c["height"], threads=threads)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(st["paths"] / dt / 1e6, 3), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port", "sample": f"the full frame ({st['paths']} paths, {dt:.1f} s)"}
     print(json.dumps(out))
 
 

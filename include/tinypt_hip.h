@@ -206,6 +206,85 @@ int tpt_hip_scene_info(const tpt_hip_scene *scene, uint64_t *arena_bytes, uint64
 int tpt_hip_render(tpt_hip_scene *scene, const tpt_render_params *params, float *out_rgb,
                    tpt_render_stats *stats);
 
+/* ---- batch: many independent renders in one pipeline ----------------------
+ * No counterpart in the reference, which renders one scene per
+ * PathTracer::render call (path_tracer.h:26-44); each scene of a batch renders
+ * exactly as tpt_hip_render would render it alone (same image, bit for bit). */
+typedef struct {
+    int32_t spp;   /* samples per pixel of this scene */
+    int32_t pad_;
+    uint64_t seed; /* stream seed of this scene */
+} tpt_batch_item;
+typedef struct {
+    double render_ms;       /* wall time of the batch render, after tuning */
+    double tune_ms;         /* wall time spent by the autotuner (0 when both sizes were given) */
+    uint64_t paths;         /* total paths of the batch */
+    int32_t queue_capacity; /* path slots per pipeline wave (tuned or given) */
+    int32_t block_size;     /* threads per workgroup of generate / shade (tuned or given) */
+    int32_t waves;          /* pipeline waves = replays of the captured hipGraph */
+    int32_t pad_;
+} tpt_batch_stats;
+/* Renders n scenes (all created on the current device with the same node layout,
+ * n <= 1024) through one shared path pool: one pipeline wave (generate, then
+ * max_depth + 1 x extend / shade / shadow, then accumulate) is captured once in
+ * a hipGraph and replayed until all paths are done.  `params` gives the shared
+ * settings (max_depth, rr_depth, shadow_bounded, output, accum_fp64); its spp and
+ * seed are ignored (per scene in `items`).  params->queue_capacity == 0 selects the
+ * capacity by autotuning over 2^18..2^22 slots, params->block_size == 0 the
+ * workgroup size over 64..512.  out_rgb[i] receives scene i's image (H*W*3
+ * floats, row 0 at the bottom); `scene_ms` (may be NULL) receives n completion
+ * times in ms since the batch started; `stats` may be NULL. */
+int tpt_hip_render_batch(tpt_hip_scene *const *scenes, const tpt_batch_item *items, int32_t n,
+                         const tpt_render_params *params, float *const *out_rgb, tpt_batch_stats *stats,
+                         double *scene_ms);
+
+/* ---- learned radiance cache --------------------------------------------
+ * No counterpart in the reference.  A 5-layer, 64-wide ReLU MLP (BF16 MFMA,
+ * FP32 accumulate) that predicts the radiance a path vertex scatters towards
+ * the camera, trained online (Adam) on path suffixes the tracer produces.  The
+ * network belongs to the scene handle and keeps training across calls. */
+typedef struct {
+    int32_t frames;        /* progressive frames of one sample per pixel */
+    int32_t cache_depth;   /* path-vertex depth at which paths terminate into the cache (2 = after two
+                              bounces); < 0: no cache (plain progressive rendering) */
+    int32_t train_records; /* training path suffixes per frame (1 .. 2^20) */
+    int32_t reset;         /* 1: re-initialise the network from weight_seed before rendering */
+    float learning_rate;   /* Adam step size */
+    int32_t train_steps;   /* Adam steps per frame, each on 1 / train_steps of the frame's records (>= 1) */
+    uint64_t weight_seed;
+} tpt_cache_params;
+typedef struct {
+    double render_ms;      /* wall time of the render */
+    double infer_ms;       /* device time of the MLP inference launches */
+    double train_ms;       /* device time of the MLP training + Adam launches */
+    double last_loss;      /* mean relative-L2 training loss of the last frame */
+    uint64_t queries;      /* paths terminated into the cache in the last frame */
+    uint64_t records;      /* training records of the last frame */
+    int32_t steps;         /* Adam steps taken so far by this scene's network */
+    int32_t pad_;
+} tpt_cache_stats;
+/* Progressive render: `cache->frames` frames of one sample per pixel (frame f
+ * uses sample index f of the per-(pixel, sample) streams of params->seed); after
+ * each frame the cache queries are evaluated, the frame's training records
+ * become targets and the network takes one Adam step.  Writes the HDR mean of
+ * the frames (H*W*3 floats, row 0 at the bottom); params->spp and output are
+ * ignored, accum_fp64 selects FP64 accumulation.  `stats` may be NULL. */
+int tpt_hip_render_cached(tpt_hip_scene *scene, const tpt_render_params *params, const tpt_cache_params *cache,
+                          float *out_rgb, tpt_cache_stats *stats);
+/* Radiance-cache test hooks.  Weights: 5 x 64 x 64 FP32, [layer][in][out] (the
+ * last layer uses out < 3); `set` / `get` may be NULL.  Setting weights resets
+ * the Adam state. */
+int tpt_hip_cache_weights(tpt_hip_scene *scene, const float *set, float *get);
+/* Evaluates the network on n host records of 64 BF16 inputs (uint16 bits):
+ * out = n x 3 floats. */
+int tpt_hip_cache_eval(tpt_hip_scene *scene, const uint16_t *inputs, int32_t n, float *out);
+/* One training step on n host records (inputs n x 64 BF16, targets n x 3
+ * floats): gradient of the mean relative-L2 loss, then Adam with step size lr.
+ * `loss` (may be NULL) receives the loss before the step; `grad` (may be NULL)
+ * receives the gradient (5 x 64 x 64 FP32, [layer][in][out]). */
+int tpt_hip_cache_train(tpt_hip_scene *scene, const uint16_t *inputs, const float *targets, int32_t n, float lr,
+                        float *loss, float *grad);
+
 /* Closest hit for n rays of 8 floats (org.xyz, tmin, dir.xyz, tmax); directions
  * must be unit length as in the reference.  Writes the global primitive id
  * (-1 on miss) and hit distance (INF on miss).  Geometric test only (no alpha

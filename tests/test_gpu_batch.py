@@ -1,0 +1,72 @@
+"""Batch mode: many independent scenes through one shared path pool and one
+captured pipeline wave.  Paths run scene-major and sample-major, so every
+pixel's samples are accumulated in sample order and each scene's image must
+equal its single-scene render bit for bit, whatever the queue capacity (pipeline
+waves that split a scene, or hold several scenes) and the workgroup size."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch(hip):
+    import zoo
+    from tinypt_amd import scenes
+
+    hosts = [
+        scenes.cornell_spheres(64, 48),
+        scenes.triangle_room(48, 40, seed=3, tris=5000),
+        zoo.feature_zoo(40, 32, importance=True),  # textures, alpha, bump, delta + env lights
+        scenes.triangle_room(32, 32, seed=4, tris=2000),
+        scenes.triangle_room(24, 56, seed=5, tris=12000),
+    ]
+    devs = [h.to("hip") for h in hosts]
+    return hosts, devs, [16, 4, 8, 3, 5], [1, 2, 3, 4, 5]
+
+
+def singles(hip, devs, spps, seeds, **kw):
+    pt = hip.PathTracer("hip")
+    return [pt.render(d, n, seed=s, **kw).raw() for d, n, s in zip(devs, spps, seeds)]
+
+
+@pytest.mark.parametrize("cap,block", [(4096, 64), (9000, 128), (1 << 18, 256)])
+def test_batch_equals_single_renders(hip, batch, cap, block):
+    hosts, devs, spps, seeds = batch
+    kw = dict(max_depth=6, rr_depth=3, hdr=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    pt = hip.PathTracer("hip")
+    imgs = pt.render_batch(devs, spps, seeds, queue_capacity=cap, block_size=block, **kw)
+    st = pt.last_batch_stats()
+    print(f"cap {cap} block {block}: {st['waves']} waves, {st['paths']} paths, {st['render_ms']:.1f} ms")
+    assert st["block_size"] == block
+    assert st["waves"] == -(-st["paths"] // st["queue_capacity"])
+    assert len(st["scene_ms"]) == len(devs) and all(t > 0 for t in st["scene_ms"])
+    for i, (a, b) in enumerate(zip(imgs, ref)):
+        assert a.raw().shape == b.shape
+        assert np.array_equal(a.raw(), b), f"scene {i} differs"
+
+
+def test_batch_ldr_fp64_and_autotune(hip, batch):
+    hosts, devs, spps, seeds = batch
+    kw = dict(max_depth=5, fp64=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    pt = hip.PathTracer("hip")
+    imgs = pt.render_batch(devs, spps, seeds, **kw)  # capacity and block size autotuned
+    st = pt.last_batch_stats()
+    print(f"tuned: cap {st['queue_capacity']} block {st['block_size']} in {st['tune_ms']:.1f} ms")
+    assert st["block_size"] in (64, 128, 256, 512)
+    assert 0 < st["queue_capacity"] <= 1 << 22
+    for i, (a, b) in enumerate(zip(imgs, ref)):
+        assert np.array_equal(a.raw(), b), f"scene {i} differs"
+
+
+def test_batch_rejects_bad_input(hip, batch):
+    hosts, devs, spps, seeds = batch
+    pt = hip.PathTracer("hip")
+    with pytest.raises(ValueError):
+        pt.render_batch(devs, spps[:-1], seeds)
+    with pytest.raises(ValueError):
+        pt.render_batch(devs[:1], [0], [1])
+    with pytest.raises(RuntimeError, match="Expect scene on hip"):
+        pt.render_batch([hosts[0]], [4], [1])

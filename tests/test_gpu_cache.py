@@ -1,0 +1,145 @@
+"""The learned radiance cache: BF16 MFMA inference and training kernels against a
+NumPy FP32 reference of the same network (weights and inter-layer activations
+rounded to BF16 as the kernels store them, FP32 accumulation), and the
+progressive renderer built on them."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    """float32 values rounded to BF16 (round to nearest even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def bits(x):
+    return (bf16(x).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def forward(W, x):
+    Wb = bf16(W)
+    acts = [x]
+    h = x
+    for ly in range(4):
+        h = bf16(np.maximum(h @ Wb[ly], 0.0))
+        acts.append(h)
+    return h @ Wb[4][:, :3], acts
+
+
+def backward(W, x, t):
+    """Loss and weight gradient of the mean relative-L2 loss, with the output and
+    layer gradients rounded to BF16 as the kernels store them."""
+    out, acts = forward(W, x)
+    n = len(x)
+    den = out * out + 0.01
+    loss = float(np.sum((out - t) ** 2 / den) / n)
+    Wb = bf16(W)
+    g = bf16(2.0 * (out - t) / den / n)
+    dW = np.zeros_like(W)
+    dW[4][:, :3] = acts[4].T @ g
+    gin = bf16((g @ Wb[4][:, :3].T) * (acts[4] > 0))
+    for ly in range(3, -1, -1):
+        dW[ly] = acts[ly].T @ gin
+        if ly > 0:
+            gin = bf16((gin @ Wb[ly].T) * (acts[ly] > 0))
+    return loss, dW
+
+
+def he_weights(rng):
+    a = np.sqrt(6.0 / 64.0)
+    W = rng.uniform(-a, a, (5, 64, 64)).astype(np.float32)
+    W[4][:, 3:] = 0.0
+    return W
+
+
+@pytest.fixture(scope="module")
+def dev(hip):
+    from tinypt_amd import scenes
+
+    return scenes.triangle_room(64, 48, seed=2, tris=5000).to("hip")
+
+
+def test_mlp_inference_matches_reference(hip, dev):
+    rng = np.random.default_rng(1)
+    W = he_weights(rng)
+    hip.cache_weights(dev, set=W)
+    for n in (1, 63, 1000, 4099):  # partial tiles and several workgroups
+        x = bf16(rng.uniform(-1, 1, (n, 64)).astype(np.float32))
+        got = hip.cache_eval(dev, bits(x))
+        ref, _ = forward(W, x)
+        scale = np.abs(ref).max() + 1e-6
+        err = np.abs(got - ref).max() / scale
+        print(f"n={n}: max |gpu - ref| / max |ref| = {err:.2e}")
+        assert np.all(np.isfinite(got))
+        assert err < 2e-2
+
+
+def test_mlp_gradient_matches_reference(hip, dev):
+    rng = np.random.default_rng(2)
+    W = he_weights(rng)
+    hip.cache_weights(dev, set=W)
+    n = 700
+    x = bf16(rng.uniform(-1, 1, (n, 64)).astype(np.float32))
+    t = rng.uniform(0.0, 2.0, (n, 3)).astype(np.float32)
+    loss, grad = hip.cache_train(dev, bits(x), t, 0.0)  # lr 0: the step leaves the weights unchanged
+    rloss, rgrad = backward(W, x, t)
+    print(f"loss gpu {loss:.6f} ref {rloss:.6f}")
+    assert abs(loss - rloss) <= 2e-3 * abs(rloss)
+    for ly in range(5):
+        a, b = grad[ly], rgrad[ly]
+        rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-12)
+        print(f"layer {ly}: |dW - ref| / |ref| = {rel:.2e}")
+        assert rel < 3e-2
+    assert np.all(grad[4][:, 3:] == 0.0)
+    assert np.array_equal(hip.cache_weights(dev), W)  # step size 0
+
+
+def test_mlp_training_fits_a_function(hip, dev):
+    rng = np.random.default_rng(3)
+    hip.cache_weights(dev, set=he_weights(rng))
+    n = 4096
+    x = bf16(rng.uniform(-1, 1, (n, 64)).astype(np.float32))
+    t = np.stack([0.6 + 0.4 * np.sin(2.0 * x[:, 0] + c) * x[:, 1 + c] for c in range(3)], 1).astype(np.float32)
+    losses = [hip.cache_train(dev, bits(x), t, 3e-3)[0] for _ in range(300)]
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert all(np.isfinite(losses))
+    assert losses[-1] < 0.2 * losses[0]
+
+
+def test_progressive_without_cache_equals_plain_render(hip, dev):
+    # frame f is sample f of the plain sampler with spp = frames (b = 1): same
+    # streams, same per-pixel FP32 summation order
+    img, st = hip.render_cached(dev, 8, cache_depth=-1, max_depth=6, rr_depth=3, seed=5)
+    ref = hip.PathTracer("hip").render(dev, 8, max_depth=6, rr_depth=3, seed=5, hdr=True).raw()
+    assert np.array_equal(img, ref)
+    assert st["queries"] == 0 and st["records"] == 0
+
+
+def test_cached_render_trains_online(hip):
+    from tinypt_amd import scenes
+
+    dev = scenes.triangle_room(128, 96).to("hip")
+    kw = dict(max_depth=8, rr_depth=5)
+    ref = hip.PathTracer("hip").render(dev, 2048, seed=99, hdr=True, fp64=True, **kw).raw()
+    plain, _ = hip.render_cached(dev, 64, cache_depth=-1, seed=1, **kw)
+    # 12,288 pixels, 2048 training records per frame (about one path in six), 4 Adam steps per frame
+    img, st = hip.render_cached(dev, 64, cache_depth=2, train_records=2048, learning_rate=1e-3, reset=True,
+                                weight_seed=7, seed=1, **kw)
+    print(st)
+    assert np.all(np.isfinite(img)) and img.min() >= 0.0
+    assert st["queries"] > 5000 and 0 < st["records"] <= 2048
+    assert st["steps"] == 64 * 4 and np.isfinite(st["last_loss"])
+
+    def rel(x):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    print(f"relMSE vs 2048 spp: cached {rel(img):.4f}, plain {rel(plain):.4f}; "
+          f"mean cached {img.mean():.4f} plain {plain.mean():.4f} ref {ref.mean():.4f}")
+    assert abs(img.mean() - ref.mean()) < 0.05 * ref.mean()  # no visible bias
+    assert rel(img) < rel(plain)  # lower error than the plain estimator at the same frames
+    # the network keeps training across calls
+    _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, **kw)
+    assert st2["steps"] == 64 * 4 + 2 * 4

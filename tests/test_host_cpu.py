@@ -72,6 +72,8 @@ def test_c_abi_rejects_invalid_input():
     assert L.tpt_hip_scene_create(None, None) == -1
     assert L.tpt_hip_render(None, None, None, None) == -1
     assert "null" in T.last_error()
+    assert L.tpt_hip_render_batch(None, None, 0, None, None, None, None) == -1
+    assert "null" in T.last_error()
 
 
 def test_material_and_light_validation():

@@ -73,6 +73,46 @@ class _RenderStats(ctypes.Structure):
     ]
 
 
+class _RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("spp", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("rr_depth", ctypes.c_int32),
+        ("shadow_bounded", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("output", ctypes.c_int32),
+        ("accum_fp64", ctypes.c_int32),
+        ("queue_capacity", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("count_traffic", ctypes.c_int32),
+    ]
+
+
+class _CacheParams(ctypes.Structure):
+    _fields_ = [
+        ("frames", ctypes.c_int32),
+        ("cache_depth", ctypes.c_int32),
+        ("train_records", ctypes.c_int32),
+        ("reset", ctypes.c_int32),
+        ("learning_rate", ctypes.c_float),
+        ("train_steps", ctypes.c_int32),
+        ("weight_seed", ctypes.c_uint64),
+    ]
+
+
+class _CacheStats(ctypes.Structure):
+    _fields_ = [
+        ("render_ms", ctypes.c_double),
+        ("infer_ms", ctypes.c_double),
+        ("train_ms", ctypes.c_double),
+        ("last_loss", ctypes.c_double),
+        ("queries", ctypes.c_uint64),
+        ("records", ctypes.c_uint64),
+        ("steps", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+    ]
+
+
 # every symbol include/tinypt_hip.h declares
 C_ABI_SYMBOLS = (
     "tpt_hip_abi_version",
@@ -85,6 +125,11 @@ C_ABI_SYMBOLS = (
     "tpt_hip_trace_closest",
     "tpt_hip_trace_occluded",
     "tpt_hip_last_trace_counters",
+    "tpt_hip_render_batch",
+    "tpt_hip_render_cached",
+    "tpt_hip_cache_weights",
+    "tpt_hip_cache_eval",
+    "tpt_hip_cache_train",
 )
 
 
@@ -105,6 +150,12 @@ def lib():
     L.tpt_hip_trace_closest.argtypes = [vp, vp, u64, vp, vp, u32, vp]
     L.tpt_hip_trace_occluded.argtypes = [vp, vp, u64, vp, u32, vp]
     L.tpt_hip_last_trace_counters.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.tpt_hip_render_batch.argtypes = [ctypes.POINTER(vp), vp, i32, vp, ctypes.POINTER(vp), vp, ctypes.POINTER(ctypes.c_double)]
+    L.tpt_hip_render_cached.argtypes = [vp, ctypes.POINTER(_RenderParams), ctypes.POINTER(_CacheParams), vp,
+                                        ctypes.POINTER(_CacheStats)]
+    L.tpt_hip_cache_weights.argtypes = [vp, vp, vp]
+    L.tpt_hip_cache_eval.argtypes = [vp, vp, i32, vp]
+    L.tpt_hip_cache_train.argtypes = [vp, vp, vp, i32, ctypes.c_float, vp, vp]
     for name in C_ABI_SYMBOLS:
         if name not in ("tpt_hip_abi_version", "tpt_hip_device_count", "tpt_hip_last_error", "tpt_hip_scene_destroy"):
             getattr(L, name).restype = ctypes.c_int
@@ -159,3 +210,60 @@ def scene_info(scene_hip):
     _check(lib().tpt_hip_scene_info(scene_hip._hip_ptr(), *[ctypes.byref(x) for x in v]))
     return dict(arena_bytes=v[0].value, nodes=v[1].value, triangles=v[2].value, analytic=v[3].value,
                 node_bytes=v[4].value)
+
+
+# ---------------------------------------------------------------- learned radiance cache
+def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3, reset=False,
+                  weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=4):
+    """Progressive render with the learned radiance cache (cache_depth < 0: plain
+    progressive frames).  Returns (H x W x 3 float32 HDR mean, row 0 at the
+    bottom, stats dict)."""
+    import numpy as np
+
+    cam = scene_hip.camera()
+    out = np.empty((cam.height, cam.width, 3), dtype=np.float32)
+    p = _RenderParams(spp=1, max_depth=max_depth, rr_depth=rr_depth, shadow_bounded=int(shadow_bounded), seed=seed,
+                      output=1, accum_fp64=int(fp64))
+    c = _CacheParams(frames=frames, cache_depth=cache_depth, train_records=train_records, reset=int(reset),
+                     learning_rate=learning_rate, train_steps=train_steps, weight_seed=weight_seed)
+    st = _CacheStats()
+    _check(lib().tpt_hip_render_cached(scene_hip._hip_ptr(), ctypes.byref(p), ctypes.byref(c), out.ctypes.data,
+                                       ctypes.byref(st)))
+    return out, {f: getattr(st, f) for f, _ in _CacheStats._fields_ if f != "pad_"}
+
+
+def cache_weights(scene_hip, set=None):
+    """Returns the cache network's FP32 weights (5, 64, 64) [layer][in][out];
+    `set` replaces them first (and resets the Adam state)."""
+    import numpy as np
+
+    get = np.empty((5, 64, 64), dtype=np.float32)
+    src = None
+    if set is not None:
+        src = np.ascontiguousarray(set, dtype=np.float32)
+        assert src.shape == (5, 64, 64)
+    _check(lib().tpt_hip_cache_weights(scene_hip._hip_ptr(), None if src is None else src.ctypes.data, get.ctypes.data))
+    return get
+
+
+def cache_eval(scene_hip, inputs_bf16):
+    """Network output (n, 3) for (n, 64) inputs given as BF16 bit patterns (uint16)."""
+    import numpy as np
+
+    x = np.ascontiguousarray(inputs_bf16, dtype=np.uint16)
+    out = np.empty((x.shape[0], 3), dtype=np.float32)
+    _check(lib().tpt_hip_cache_eval(scene_hip._hip_ptr(), x.ctypes.data, x.shape[0], out.ctypes.data))
+    return out
+
+
+def cache_train(scene_hip, inputs_bf16, targets, lr):
+    """One training step; returns (loss before the step, gradient (5, 64, 64))."""
+    import numpy as np
+
+    x = np.ascontiguousarray(inputs_bf16, dtype=np.uint16)
+    t = np.ascontiguousarray(targets, dtype=np.float32)
+    loss = ctypes.c_float()
+    grad = np.empty((5, 64, 64), dtype=np.float32)
+    _check(lib().tpt_hip_cache_train(scene_hip._hip_ptr(), x.ctypes.data, t.ctypes.data, x.shape[0], lr,
+                                     ctypes.byref(loss), grad.ctypes.data))
+    return loss.value, grad

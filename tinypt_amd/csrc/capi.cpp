@@ -11,25 +11,35 @@
 #include <cstdlib>
 #include <chrono>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
 #include "../../include/tinypt_hip.h"
 #include "dev_math.h"
 #include "dev_types.h"
+#include "nrc.h"
 #include "scene_build.h"
 
 namespace tpt {
 void launch_generate(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, hipStream_t);
 void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, hipStream_t);
 void launch_shade(const DScene &, const DRender &, const DPaths &, const int32_t *, int32_t *, int, int, int, int,
-                  hipStream_t);
+                  hipStream_t, const DCache *);
 void launch_shadow(const DScene &, const DPaths &, int, int, bool, hipStream_t);
 void launch_accumulate(const DRender &, const DPaths &, int, int, int, hipStream_t);
 void launch_finalize(const DRender &, const DPaths &, float *, hipStream_t);
 void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, float *, uint32_t *, bool, int, int,
                           hipStream_t);
 void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
+void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, int, uint64_t, hipStream_t);
+void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
+void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
+void launch_cache_reset(const DCache &, hipStream_t);
+void launch_cache_apply(const DPaths &, const DCache &, const float4 *, int, hipStream_t);
+void launch_cache_targets(const DPaths &, const DCache &, float4 *, hipStream_t);
+void launch_frame_accumulate(const DPaths &, int, int, int, hipStream_t);
+void launch_frame_finalize(const DPaths &, int, int, int, float *, hipStream_t);
 } // namespace tpt
 
 using namespace tpt;
@@ -74,6 +84,25 @@ struct PathBuffers {
     float *out = nullptr;
 };
 
+// Radiance-cache state of a scene handle: the network (weights and Adam state,
+// kept across renders: the cache trains online) and the per-slot staging buffers.
+struct CacheState {
+    void *wmem = nullptr; // weights, moments, gradient, BF16 copies, loss, counters
+    NrcWeights W{};
+    uint32_t *counters = nullptr;
+    int step = 0;
+    void *bmem = nullptr; // per-slot buffers for `slots` paths
+    size_t slots = 0;
+    int train_cap = 0;
+    uint16_t *feat = nullptr;
+    float4 *rec_T = nullptr, *rec_C = nullptr, *pred = nullptr, *target = nullptr;
+    int32_t *q_slot = nullptr, *t_slot = nullptr;
+    ~CacheState() {
+        if (wmem) (void)hipFree(wmem);
+        if (bmem) (void)hipFree(bmem);
+    }
+};
+
 } // namespace
 
 struct tpt_hip_scene {
@@ -87,7 +116,10 @@ struct tpt_hip_scene {
     uint32_t *trace_counters = nullptr; // 8 x u32
     uint64_t last_nodes = 0, last_tris = 0;
     hipStream_t stream = nullptr;
+    float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
+    CacheState cache;
     ~tpt_hip_scene() {
+        if (stream) (void)hipStreamSynchronize(stream);
         if (pb.mem) (void)hipFree(pb.mem);
         if (arena) (void)hipFree(arena);
         if (trace_counters) (void)hipFree(trace_counters);
@@ -183,11 +215,41 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     h->num_nodes = H.root >= 0 && !(H.root & (int32_t)kLeafBit) ? (uint64_t)S.num_nodes : 0;
     h->num_tris = H.root == -1 ? 0 : (uint64_t)S.num_tris;
     h->num_analytic = H.analytic.size();
+    // bounds of the finite geometry: triangles, and analytic objects of radius /
+    // extent below 1e4 (the radius-1e5 wall spheres of smallpt-style scenes are
+    // left out so that the position encoding keeps its resolution)
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    auto grow = [&](float x, float y, float z, float r) {
+        const float p[3] = {x, y, z};
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a] - r);
+            hi[a] = std::max(hi[a], p[a] + r);
+        }
+    };
+    if (H.root != -1)
+        for (size_t t = 0; t + 2 < H.tris.size(); t += 3) {
+            float4 a = H.tris[t], ab = H.tris[t + 1], ac = H.tris[t + 2];
+            grow(a.x, a.y, a.z, 0.f);
+            grow(a.x + ab.x, a.y + ab.y, a.z + ab.z, 0.f);
+            grow(a.x + ac.x, a.y + ac.y, a.z + ac.z, 0.f);
+        }
+    for (const DAnalytic &an2 : H.analytic) {
+        float r = an2.type == A_SPHERE || an2.type == A_CIRCLE ? an2.radius : std::max(an2.dim[0], an2.dim[1]);
+        if (r < 1e4f) grow(an2.center[0], an2.center[1], an2.center[2], r);
+    }
+    for (int a = 0; a < 3; a++) {
+        if (!(lo[a] <= hi[a])) {
+            lo[a] = -1.f;
+            hi[a] = 1.f;
+        }
+        h->bounds_lo[a] = lo[a];
+        h->bounds_hi[a] = std::max(hi[a], lo[a] + 1e-3f);
+    }
 }
 
-void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
-    PathBuffers &pb = h->pb;
-    if (pb.mem && pb.slots >= slots && pb.pixels == pixels) return;
+// Path pool of `slots` paths + accumulators / output of `pixels` pixels (one
+// allocation).  `with_sid`: also the per-slot scene ids of batch mode.
+void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
     if (pb.mem) {
         HIP_OK(hipFree(pb.mem));
         pb.mem = nullptr;
@@ -205,12 +267,14 @@ void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
     size_t o_cnt = take(64);
     size_t o_sf = take((size_t)pixels * 12), o_sdd = take((size_t)pixels * 24), o_pf = take((size_t)pixels * 12),
            o_pd = take((size_t)pixels * 24), o_out = take((size_t)pixels * 12);
+    size_t o_sid = with_sid ? take(slots * 4) : 0;
     HIP_OK(hipMalloc(&pb.mem, off));
     pb.bytes = off;
     pb.slots = slots;
-    pb.pixels = pixels;
+    pb.pixels = (int)std::min<size_t>(pixels, (size_t)INT32_MAX);
     uint8_t *b = (uint8_t *)pb.mem;
     DPaths &P = pb.P;
+    P.sid = with_sid ? (int32_t *)(b + o_sid) : nullptr;
     P.org = (float4 *)(b + o_org);
     P.dir = (float4 *)(b + o_dir);
     P.decay = (float4 *)(b + o_dec);
@@ -234,12 +298,34 @@ void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
     HIP_OK(hipMemset(P.counters, 0, 64));
 }
 
+void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
+    PathBuffers &pb = h->pb;
+    if (pb.mem && pb.slots >= slots && pb.pixels == pixels) return;
+    alloc_paths(pb, slots, (size_t)pixels, false);
+}
+
 int block_size_for(int spp) { // PixelSampler ctor (cpu/path_tracer.cpp:10-21)
     const int m = 16;
     if (spp < 2 * 2 * m) return 1;
     if (spp < 3 * 3 * m) return 2;
     if (spp < 4 * 4 * m) return 3;
     return 4;
+}
+
+// Sampler and integrator settings of one render (cpu/path_tracer.cpp:8-21, 102-104).
+DRender make_render(const tpt_render_params *p, int spp, uint64_t seed, int pixels) {
+    DRender R{};
+    R.block = block_size_for(spp);
+    R.nsub = spp / (R.block * R.block);
+    R.spp_eff = R.nsub * R.block * R.block;
+    R.seed = seed;
+    R.max_depth = p->max_depth;
+    R.rr_depth = p->rr_depth;
+    R.shadow_bounded = p->shadow_bounded;
+    R.output = p->output == TPT_OUT_HDR_MEAN ? 1 : 0;
+    R.accum_fp64 = p->accum_fp64 ? 1 : 0;
+    R.pixels = pixels;
+    return R;
 }
 
 int grid_for(size_t n, int block, int cus = 256) {
@@ -325,17 +411,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         const DScene &S = h->S;
         int W = S.cam.width, Hh = S.cam.height;
         int pixels = W * Hh;
-        DRender R{};
-        R.block = block_size_for(p->spp);
-        R.nsub = p->spp / (R.block * R.block);
-        R.spp_eff = R.nsub * R.block * R.block;
-        R.seed = p->seed;
-        R.max_depth = p->max_depth;
-        R.rr_depth = p->rr_depth;
-        R.shadow_bounded = p->shadow_bounded;
-        R.output = p->output == TPT_OUT_HDR_MEAN ? 1 : 0;
-        R.accum_fp64 = p->accum_fp64 ? 1 : 0;
-        R.pixels = pixels;
+        DRender R = make_render(p, p->spp, p->seed, pixels);
         size_t cap = p->queue_capacity > 0 ? (size_t)p->queue_capacity : ((size_t)1 << 24);
         int k = (int)std::max<size_t>(1, std::min<size_t>((size_t)R.spp_eff, cap / (size_t)pixels));
         R.k = k;
@@ -370,7 +446,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
                 int e0 = timing ? ev() : -1;
                 launch_extend(S, P, q, cur, 1 - cur, grid, block, count, st);
                 int e1 = timing ? ev() : -1;
-                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st);
+                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, nullptr);
                 int e2 = timing ? ev() : -1;
                 launch_shadow(S, P, grid, block, count, st);
                 int e3 = timing ? ev() : -1;
@@ -421,6 +497,544 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         return set_err(e.code, e.what());
     } catch (const std::exception &e) {
         return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+namespace {
+
+// Device state of one batch render: scene / sampler tables, segment table and
+// the shared path pool.
+struct BatchRun {
+    void *meta = nullptr;
+    PathBuffers pb;
+    DBatch B{};
+    uint64_t total_paths = 0, total_pixels = 0;
+    std::vector<uint64_t> path_off, pix_off;
+    int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
+    hipStream_t st = nullptr;
+    ~BatchRun() {
+        if (st) (void)hipStreamSynchronize(st); // nothing queued may still use the buffers
+        if (pb.mem) (void)hipFree(pb.mem);
+        if (meta) (void)hipFree(meta);
+    }
+    // capture one pipeline wave of (cap, block) into an executable graph
+    hipGraphExec_t capture(int cap, int block) {
+        B.cap = cap;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, num_cu, total_pixels, st);
+        HIP_OK(hipStreamEndCapture(st, &g));
+        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_OK(e);
+        return ge;
+    }
+    // restart at the first path with zeroed accumulators (the stream is drained
+    // first: the host copy is not ordered with the non-blocking stream)
+    void reset() {
+        uint64_t st0[2] = {0, total_paths};
+        HIP_OK(hipStreamSynchronize(st));
+        HIP_OK(hipMemcpy(B.state, st0, sizeof(st0), hipMemcpyHostToDevice));
+        launch_batch_zero(B, pb.P, fp64, total_pixels, st);
+    }
+    int waves(int cap) const { return (int)((total_paths + (uint64_t)cap - 1) / (uint64_t)cap); }
+};
+
+// Paths per ms of (cap, block) over the first few pipeline waves (the first
+// replay warms the caches and is not timed when more than one runs).
+double tune_trial(BatchRun &run, int cap, int block) {
+    hipGraphExec_t ge = run.capture(cap, block);
+    run.reset();
+    int reps = std::min(run.waves(cap), 3);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    if (reps > 1) HIP_OK(hipGraphLaunch(ge, run.st));
+    HIP_OK(hipEventRecord(e0, run.st));
+    uint64_t done = reps > 1 ? (uint64_t)cap : 0, paths = 0;
+    for (int r = (reps > 1 ? 1 : 0); r < reps; r++) {
+        HIP_OK(hipGraphLaunch(ge, run.st));
+        paths += std::min<uint64_t>((uint64_t)cap, run.total_paths - done);
+        done += (uint64_t)cap;
+    }
+    HIP_OK(hipEventRecord(e1, run.st));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ge);
+    return ms > 0 ? (double)paths / (double)ms : 0.0;
+}
+
+} // namespace
+
+int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *items_in, int32_t n,
+                         const tpt_render_params *p, float *const *out_in, tpt_batch_stats *stats,
+                         double *scene_ms) {
+    if (!scenes_in || !items_in || !p || !out_in || n <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < n; i++)
+        if (!scenes_in[i]) return set_err(TPT_ERR_INVALID_ARGUMENT, "null scene or output");
+    // Paths run shortest scene first (fewest paths), which minimises the mean
+    // completion time; the images do not depend on the order.
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int i) {
+        const DScene &S = scenes_in[i]->S;
+        int b = block_size_for(std::max(1, items_in[i].spp));
+        return (uint64_t)S.cam.width * (uint64_t)S.cam.height * (uint64_t)(items_in[i].spp / (b * b) * b * b);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
+    std::vector<tpt_hip_scene *> scenes(n);
+    std::vector<tpt_batch_item> items(n);
+    std::vector<float *> out_rgb(n);
+    for (int k = 0; k < n; k++) {
+        scenes[k] = scenes_in[order[k]];
+        items[k] = items_in[order[k]];
+        out_rgb[k] = out_in[order[k]];
+    }
+    if (n > 1024) return set_err(TPT_ERR_INVALID_ARGUMENT, "at most 1024 scenes per batch");
+    if (p->max_depth < 0 || p->max_depth > 1000) return set_err(TPT_ERR_INVALID_ARGUMENT, "max_depth out of range");
+    if (p->block_size != 0 && (p->block_size % 64 != 0 || p->block_size > 1024))
+        return set_err(TPT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 <= 1024");
+    if (p->queue_capacity < 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "queue_capacity must not be negative");
+    for (int i = 0; i < n; i++) {
+        if (!scenes[i] || !out_rgb[i]) return set_err(TPT_ERR_INVALID_ARGUMENT, "null scene or output");
+        if (items[i].spp <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "spp must be positive");
+        if (scenes[i]->device != scenes[0]->device) return set_err(TPT_ERR_INVALID_ARGUMENT, "scenes on different devices");
+        if (scenes[i]->S.layout != scenes[0]->S.layout)
+            return set_err(TPT_ERR_INVALID_ARGUMENT, "scenes with different BVH node layouts");
+    }
+    try {
+        auto t0 = std::chrono::steady_clock::now();
+        HIP_OK(hipSetDevice(scenes[0]->device));
+        BatchRun run;
+        run.st = scenes[0]->stream;
+        run.max_depth = p->max_depth;
+        run.layout = scenes[0]->S.layout;
+        run.trav_block = scenes[0]->S.trav_block;
+        run.num_cu = scenes[0]->cus;
+        run.fp64 = p->accum_fp64 ? 1 : 0;
+        std::vector<DScene> table(n);
+        std::vector<DRender> rend(n);
+        std::vector<uint64_t> seeds(n);
+        run.path_off.assign(n + 1, 0);
+        run.pix_off.assign(n + 1, 0);
+        for (int i = 0; i < n; i++) {
+            table[i] = scenes[i]->S;
+            table[i].lds_nodes = 0; // no LDS node cache: wavefronts of one workgroup trace different scenes
+            table[i].lds_stack = 0;
+            int px = table[i].cam.width * table[i].cam.height;
+            rend[i] = make_render(p, items[i].spp, items[i].seed, px);
+            seeds[i] = splitmix64(items[i].seed);
+            run.path_off[i + 1] = run.path_off[i] + (uint64_t)px * (uint64_t)rend[i].spp_eff;
+            run.pix_off[i + 1] = run.pix_off[i] + (uint64_t)px;
+        }
+        run.total_paths = run.path_off[n];
+        run.total_pixels = run.pix_off[n];
+        // tables in one allocation
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            size_t o = off;
+            off = align256(off + bytes);
+            return o;
+        };
+        size_t o_sc = take(sizeof(DScene) * n), o_r = take(sizeof(DRender) * n), o_po = take(8 * (n + 1)),
+               o_xo = take(8 * (n + 1)), o_sd = take(8 * n), o_sg = take(4 * (size_t)kSegFields * n), o_st = take(16);
+        HIP_OK(hipMalloc(&run.meta, off));
+        uint8_t *m = (uint8_t *)run.meta;
+        HIP_OK(hipMemcpy(m + o_sc, table.data(), sizeof(DScene) * n, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m + o_r, rend.data(), sizeof(DRender) * n, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m + o_po, run.path_off.data(), 8 * (n + 1), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m + o_xo, run.pix_off.data(), 8 * (n + 1), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m + o_sd, seeds.data(), 8 * n, hipMemcpyHostToDevice));
+        HIP_OK(hipMemsetAsync(m + o_sg, 0, 4 * (size_t)kSegFields * n, run.st));
+        DBatch &B = run.B;
+        B.scenes = (const DScene *)(m + o_sc);
+        B.renders = (const DRender *)(m + o_r);
+        B.path_off = (const uint64_t *)(m + o_po);
+        B.pix_off = (const uint64_t *)(m + o_xo);
+        B.seed_mixed = (const uint64_t *)(m + o_sd);
+        B.seg = (int32_t *)(m + o_sg);
+        B.state = (uint64_t *)(m + o_st);
+        B.n = n;
+        // candidate capacities: 2^18 .. 2^22 path slots, never (much) more than the batch needs
+        auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
+        std::vector<int> caps;
+        if (p->queue_capacity > 0) caps.push_back(round_cap((uint64_t)p->queue_capacity));
+        else
+            for (int e = 18; e <= 22; e++) {
+                int c = round_cap(1ull << e);
+                if (caps.empty() || caps.back() != c) caps.push_back(c);
+            }
+        std::vector<int> blocks;
+        if (p->block_size > 0) blocks.push_back(p->block_size);
+        else blocks = {64, 128, 256, 512};
+        alloc_paths(run.pb, (size_t)*std::max_element(caps.begin(), caps.end()), (size_t)run.total_pixels, true);
+        int best_cap = caps[0], best_block = blocks.size() == 1 ? blocks[0] : 256;
+        double tune_ms = 0;
+        if (caps.size() > 1 || blocks.size() > 1) {
+            auto tt = std::chrono::steady_clock::now();
+            double best = -1;
+            for (int c : caps) { // capacity first, at the middle block size
+                double r = tune_trial(run, c, best_block);
+                if (r > best) {
+                    best = r;
+                    best_cap = c;
+                }
+            }
+            if (blocks.size() > 1) {
+                for (int b : blocks) {
+                    if (b == 256) continue; // measured above
+                    double r = tune_trial(run, best_cap, b);
+                    if (r > best) {
+                        best = r;
+                        best_block = b;
+                    }
+                }
+            }
+            tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
+        }
+        // the render: one captured pipeline wave, replayed until every path is done
+        auto tr = std::chrono::steady_clock::now();
+        hipGraphExec_t ge = run.capture(best_cap, best_block);
+        run.reset();
+        int W = run.waves(best_cap);
+        std::vector<hipEvent_t> ev(W + 1);
+        for (auto &e : ev) HIP_OK(hipEventCreate(&e));
+        HIP_OK(hipEventRecord(ev[0], run.st));
+        for (int w = 0; w < W; w++) {
+            HIP_OK(hipGraphLaunch(ge, run.st));
+            HIP_OK(hipEventRecord(ev[w + 1], run.st));
+        }
+        launch_batch_finalize(B, run.pb.P, run.pb.out, run.total_pixels, run.st);
+        HIP_OK(hipGetLastError());
+        for (int i = 0; i < n; i++)
+            HIP_OK(hipMemcpyAsync(out_rgb[i], run.pb.out + 3 * run.pix_off[i], (run.pix_off[i + 1] - run.pix_off[i]) * 12,
+                                  hipMemcpyDeviceToHost, run.st));
+        HIP_OK(hipStreamSynchronize(run.st));
+        auto t1 = std::chrono::steady_clock::now();
+        if (scene_ms) {
+            for (int i = 0; i < n; i++) {
+                int w = (int)((run.path_off[i + 1] - 1) / (uint64_t)best_cap);
+                float ms = 0;
+                HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[w + 1]));
+                scene_ms[order[i]] = ms;
+            }
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        (void)hipGraphExecDestroy(ge);
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->render_ms = std::chrono::duration<double, std::milli>(t1 - tr).count();
+            stats->tune_ms = tune_ms;
+            stats->paths = run.total_paths;
+            stats->queue_capacity = best_cap;
+            stats->block_size = best_block;
+            stats->waves = W;
+        }
+        (void)t0;
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+namespace {
+
+// Network storage of a scene handle (allocated once).
+void cache_alloc_weights(tpt_hip_scene *h) {
+    CacheState &c = h->cache;
+    if (c.wmem) return;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align256(off + bytes);
+        return o;
+    };
+    size_t o_w = take(4 * kNrcParams), o_m = take(4 * kNrcParams), o_v = take(4 * kNrcParams),
+           o_g = take(4 * kNrcParams), o_wt = take(2 * kNrcParams), o_wb = take(2 * kNrcParams), o_l = take(16),
+           o_c = take(16);
+    HIP_OK(hipMalloc(&c.wmem, off));
+    HIP_OK(hipMemset(c.wmem, 0, off));
+    uint8_t *b = (uint8_t *)c.wmem;
+    c.W.w = (float *)(b + o_w);
+    c.W.m = (float *)(b + o_m);
+    c.W.v = (float *)(b + o_v);
+    c.W.grad = (float *)(b + o_g);
+    c.W.wt = (uint16_t *)(b + o_wt);
+    c.W.wb = (uint16_t *)(b + o_wb);
+    c.W.loss = (float *)(b + o_l);
+    c.counters = (uint32_t *)(b + o_c);
+}
+
+// Uploads FP32 weights [l][in][out] with their BF16 copies and clears the Adam state.
+void cache_set_weights(tpt_hip_scene *h, const float *w) {
+    CacheState &c = h->cache;
+    std::vector<float> wf(w, w + kNrcParams);
+    std::vector<uint16_t> wt(kNrcParams), wb(kNrcParams);
+    for (int ly = 0; ly < 5; ly++)
+        for (int in = 0; in < 64; in++)
+            for (int out = 0; out < 64; out++) {
+                int i = ly * 4096 + in * 64 + out;
+                if (ly == 4 && out >= 3) wf[i] = 0.f;
+                uint16_t b = f32_to_bf16_bits(wf[i]);
+                wb[i] = b;
+                wt[ly * 4096 + out * 64 + in] = b;
+            }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(c.W.w, wf.data(), 4 * kNrcParams, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c.W.wt, wt.data(), 2 * kNrcParams, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c.W.wb, wb.data(), 2 * kNrcParams, hipMemcpyHostToDevice));
+    std::vector<float> zero(kNrcParams, 0.f);
+    HIP_OK(hipMemcpy(c.W.m, zero.data(), 4 * kNrcParams, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c.W.v, zero.data(), 4 * kNrcParams, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c.W.grad, zero.data(), 4 * kNrcParams, hipMemcpyHostToDevice));
+    c.step = 0;
+}
+
+// Hidden layers: He-uniform U(-sqrt(6 / 64), sqrt(6 / 64)), seeded; the output
+// layer starts at zero, so an untrained cache predicts no radiance.
+void cache_init_weights(tpt_hip_scene *h, uint64_t seed) {
+    std::vector<float> w(kNrcParams, 0.f);
+    uint64_t s = seed;
+    const float a = std::sqrt(6.f / 64.f);
+    for (int i = 0; i < 4 * 4096; i++) {
+        s = splitmix64(s + 0x9E3779B97F4A7C15ull);
+        float u = (float)((s >> 40) * (1.0 / 16777216.0)); // [0, 1)
+        w[i] = (2.f * u - 1.f) * a;
+    }
+    cache_set_weights(h, w.data());
+}
+
+// Per-slot staging buffers for `slots` paths and `train_cap` training records.
+void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
+    CacheState &c = h->cache;
+    if (c.bmem && c.slots >= slots && c.train_cap >= train_cap) return;
+    if (c.bmem) {
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipFree(c.bmem));
+        c.bmem = nullptr;
+    }
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = align256(off + bytes);
+        return o;
+    };
+    size_t o_f = take(slots * kNrcInputs * 2), o_T = take(slots * 16), o_C = take(slots * 16), o_p = take(slots * 16),
+           o_q = take(slots * 4), o_t = take((size_t)train_cap * 4), o_tg = take((size_t)train_cap * 16);
+    HIP_OK(hipMalloc(&c.bmem, off));
+    uint8_t *b = (uint8_t *)c.bmem;
+    c.feat = (uint16_t *)(b + o_f);
+    c.rec_T = (float4 *)(b + o_T);
+    c.rec_C = (float4 *)(b + o_C);
+    c.pred = (float4 *)(b + o_p);
+    c.q_slot = (int32_t *)(b + o_q);
+    c.t_slot = (int32_t *)(b + o_t);
+    c.target = (float4 *)(b + o_tg);
+    c.slots = slots;
+    c.train_cap = train_cap;
+}
+
+} // namespace
+
+int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_params *cp, float *out_rgb,
+                          tpt_cache_stats *stats) {
+    if (!h || !p || !cp || !out_rgb) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (cp->frames <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "frames must be positive");
+    if (cp->train_records <= 0 || cp->train_records > (1 << 20))
+        return set_err(TPT_ERR_INVALID_ARGUMENT, "train_records must be in 1 .. 2^20");
+    if (p->max_depth < 0 || p->max_depth > 1000) return set_err(TPT_ERR_INVALID_ARGUMENT, "max_depth out of range");
+    if (!(cp->learning_rate >= 0.f)) return set_err(TPT_ERR_INVALID_ARGUMENT, "learning_rate must not be negative");
+    if (cp->train_steps < 1 || cp->train_steps > cp->train_records)
+        return set_err(TPT_ERR_INVALID_ARGUMENT, "train_steps must be in 1 .. train_records");
+    try {
+        auto t0 = std::chrono::steady_clock::now();
+        HIP_OK(hipSetDevice(h->device));
+        const DScene &S = h->S;
+        const int pixels = S.cam.width * S.cam.height;
+        const bool use_cache = cp->cache_depth >= 0;
+        // one sample per pixel and frame; frame f is sample f of the plain sampler
+        // with spp = frames and no sub-pixel strata (b = 1, nsub = frames)
+        DRender R = make_render(p, 1, p->seed, pixels);
+        R.block = 1;
+        R.nsub = cp->frames;
+        R.spp_eff = cp->frames;
+        R.output = 1;
+        R.k = 1;
+        hipStream_t st = h->stream;
+        ensure_paths(h, (size_t)pixels, pixels);
+        DPaths &P = h->pb.P;
+        cache_alloc_weights(h);
+        if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
+        CacheState &c = h->cache;
+        cache_alloc_buffers(h, (size_t)pixels, cp->train_records);
+        DCache C{};
+        C.depth = cp->cache_depth;
+        // enough training candidates that about train_records of them reach the cache vertex
+        C.train_stride = std::max(1, (int)((int64_t)pixels * 2 / (3 * (int64_t)cp->train_records)));
+        C.train_cap = cp->train_records;
+        for (int a = 0; a < 3; a++) {
+            C.lo[a] = h->bounds_lo[a];
+            C.inv_ext[a] = 1.f / (h->bounds_hi[a] - h->bounds_lo[a]);
+        }
+        C.feat = c.feat;
+        C.rec_T = c.rec_T;
+        C.rec_C = c.rec_C;
+        C.q_slot = c.q_slot;
+        C.t_slot = c.t_slot;
+        C.counters = c.counters;
+        const int block = 256;
+        const int grid = grid_for((size_t)pixels, block, h->cus);
+        const uint64_t base_mixed = splitmix64(p->seed);
+        const int fp64 = p->accum_fp64 ? 1 : 0;
+        hipEvent_t e[4];
+        for (auto &x : e) HIP_OK(hipEventCreate(&x));
+        double infer_ms = 0, train_ms = 0;
+        for (int f = 0; f < cp->frames; f++) {
+            C.frame = (uint32_t)(c.step + f);
+            launch_cache_reset(C, st);
+            launch_generate(S, R, P, f, pixels, base_mixed, block, st);
+            int cur = 0;
+            for (int b = 0; b <= R.max_depth; b++) {
+                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
+                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, false, st);
+                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, use_cache ? &C : nullptr);
+                launch_shadow(S, P, grid, block, false, st);
+                cur = 1 - cur;
+            }
+            if (use_cache) {
+                bool last = f + 1 == cp->frames;
+                if (last) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
+                HIP_OK(hipEventRecord(e[0], st));
+                nrc_launch_infer(c.feat, c.q_slot, c.counters + 0, pixels, c.W.wt, c.pred, st);
+                HIP_OK(hipEventRecord(e[1], st));
+                launch_cache_apply(P, C, c.pred, pixels, st);
+                launch_cache_targets(P, C, c.target, st);
+                HIP_OK(hipEventRecord(e[2], st));
+                // train_steps Adam steps on consecutive mini-batches of the frame's records
+                const int K = cp->train_steps, mb = (cp->train_records + K - 1) / K;
+                for (int k = 0; k < K; k++) {
+                    nrc_launch_train(c.feat, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
+                    nrc_launch_adam(c.W, cp->learning_rate, ++c.step, st);
+                }
+                HIP_OK(hipEventRecord(e[3], st));
+                HIP_OK(hipEventSynchronize(e[3]));
+                float a = 0, b2 = 0;
+                HIP_OK(hipEventElapsedTime(&a, e[0], e[1]));
+                HIP_OK(hipEventElapsedTime(&b2, e[2], e[3]));
+                infer_ms += a;
+                train_ms += b2;
+            }
+            launch_frame_accumulate(P, pixels, fp64, f == 0, st);
+        }
+        launch_frame_finalize(P, pixels, fp64, cp->frames, h->pb.out, st);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
+        uint32_t cnt[2] = {0, 0};
+        float loss = 0.f;
+        HIP_OK(hipMemcpyAsync(cnt, c.counters, 8, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        for (auto &x : e) (void)hipEventDestroy(x);
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            stats->infer_ms = infer_ms;
+            stats->train_ms = train_ms;
+            stats->last_loss = use_cache ? loss / cp->train_steps : 0.0; // sum of the mini-batch means
+            stats->queries = use_cache ? cnt[0] : 0;
+            stats->records = use_cache ? std::min<uint32_t>(cnt[1], (uint32_t)cp->train_records) : 0;
+            stats->steps = c.step;
+        }
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+int tpt_hip_cache_weights(tpt_hip_scene *h, const float *set, float *get) {
+    if (!h) return set_err(TPT_ERR_INVALID_ARGUMENT, "null scene");
+    try {
+        HIP_OK(hipSetDevice(h->device));
+        cache_alloc_weights(h);
+        if (set) cache_set_weights(h, set);
+        else if (h->cache.step == 0 && !get) cache_init_weights(h, 0);
+        if (get) {
+            HIP_OK(hipStreamSynchronize(h->stream));
+            HIP_OK(hipMemcpy(get, h->cache.W.w, 4 * kNrcParams, hipMemcpyDeviceToHost));
+        }
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    }
+}
+
+namespace {
+// Host records -> device staging (rows 0..n-1 of the feature buffer, targets).
+void cache_stage(tpt_hip_scene *h, const uint16_t *inputs, const float *targets, int n) {
+    cache_alloc_weights(h);
+    cache_alloc_buffers(h, (size_t)std::max(n, 64), std::max(n, 64));
+    CacheState &c = h->cache;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(c.feat, inputs, (size_t)n * kNrcInputs * 2, hipMemcpyHostToDevice));
+    if (targets) {
+        std::vector<float4> t(n);
+        for (int i = 0; i < n; i++) t[i] = make_float4(targets[3 * i], targets[3 * i + 1], targets[3 * i + 2], 0.f);
+        HIP_OK(hipMemcpy(c.target, t.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    }
+    uint32_t cnt[2] = {(uint32_t)n, (uint32_t)n};
+    HIP_OK(hipMemcpy(c.counters, cnt, 8, hipMemcpyHostToDevice));
+}
+} // namespace
+
+int tpt_hip_cache_eval(tpt_hip_scene *h, const uint16_t *inputs, int32_t n, float *out) {
+    if (!h || !inputs || !out || n <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    try {
+        HIP_OK(hipSetDevice(h->device));
+        cache_stage(h, inputs, nullptr, n);
+        CacheState &c = h->cache;
+        nrc_launch_infer(c.feat, nullptr, c.counters, n, c.W.wt, c.pred, h->stream);
+        HIP_OK(hipGetLastError());
+        std::vector<float4> o(n);
+        HIP_OK(hipMemcpyAsync(o.data(), c.pred, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < n; i++) {
+            out[3 * i] = o[i].x;
+            out[3 * i + 1] = o[i].y;
+            out[3 * i + 2] = o[i].z;
+        }
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    }
+}
+
+int tpt_hip_cache_train(tpt_hip_scene *h, const uint16_t *inputs, const float *targets, int32_t n, float lr,
+                        float *loss, float *grad) {
+    if (!h || !inputs || !targets || n <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    try {
+        HIP_OK(hipSetDevice(h->device));
+        cache_stage(h, inputs, targets, n);
+        CacheState &c = h->cache;
+        hipStream_t st = h->stream;
+        HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
+        nrc_launch_train(c.feat, nullptr, c.target, c.counters + 1, 0, n, c.W, st);
+        HIP_OK(hipGetLastError());
+        if (grad) HIP_OK(hipMemcpyAsync(grad, c.W.grad, 4 * kNrcParams, hipMemcpyDeviceToHost, st));
+        if (loss) HIP_OK(hipMemcpyAsync(loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
+        nrc_launch_adam(c.W, lr, ++c.step, st);
+        HIP_OK(hipStreamSynchronize(st));
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
     }
 }
 

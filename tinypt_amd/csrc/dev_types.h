@@ -189,6 +189,53 @@ struct DPaths {
     double *sub_d;  // 3 per pixel (fp64 mode)
     float *pix_f;   // 3 per pixel: sum of clipped sub-pixel means (or hdr sum)
     double *pix_d;  // 3 per pixel (fp64 hdr)
+    int32_t *sid;   // batch mode: scene of each slot
+};
+
+// Learned radiance cache hooks of the shade kernel.  At the path vertex of depth
+// `depth`, a path either terminates into the cache (a query: its encoded vertex
+// is evaluated by the MLP afterwards and color += decay * prediction), or, for
+// one path in `train_stride`, is recorded as a training example (its throughput
+// T and radiance C at the vertex) and continues; its target is then
+// (final radiance - C) / T, the radiance the vertex scattered towards the camera.
+struct DCache {
+    int32_t depth;        // cache vertex depth (< 0: cache off)
+    int32_t train_stride; // one path in train_stride (by a per-frame hash of its slot) trains
+    uint32_t frame;       // frame index (training-path hash)
+    int32_t train_cap;    // capacity of the training-record list
+    float lo[3];          // scene bounds: position encoding (p - lo) * inv_ext
+    float inv_ext[3];
+    uint16_t *feat;       // per slot: 64 BF16 inputs of its cache vertex
+    float4 *rec_T;        // per slot: throughput arriving at the vertex (training paths)
+    float4 *rec_C;        // per slot: radiance after the vertex's emission (training paths)
+    int32_t *q_slot;      // query list (slots), count in counters[0]
+    int32_t *t_slot;      // training list (slots), count in counters[1]
+    uint32_t *counters;
+};
+
+// Batch mode: many independent scenes share one path pool and one pipeline.
+// Global path index g runs scene-major (scene 0's paths first), and within a
+// scene sample-major (g - path_off[s] = sample * pixels + pixel), so every
+// pixel's samples arrive in sample order and the accumulation order equals a
+// single-scene render's.  A pipeline wave covers global paths [g0, g0 + cap).
+// Ray and shadow queues are split into one contiguous segment per scene, so a
+// wavefront of the persistent traversal kernels works on one scene at a time.
+enum {
+    kSegQaBase = 0, kSegQaCnt, kSegQbBase, kSegQbCnt, // ping-pong ray queues
+    kSegFetch,                                        // extend refill counter
+    kSegShBase, kSegShCnt, kSegShFetch,               // shadow queue
+    kSegFields
+};
+struct DBatch {
+    const DScene *scenes;       // n
+    const DRender *renders;     // n: per-scene sampler (spp_eff, block, nsub, pixels) + shared depth / RR / output
+    const uint64_t *path_off;   // n + 1
+    const uint64_t *pix_off;    // n + 1: first pixel of each scene in the concatenated accumulators / output
+    const uint64_t *seed_mixed; // n: splitmix64(seed) of each scene
+    int32_t *seg;               // kSegFields * n, field-major: seg[f * n + s]
+    uint64_t *state;            // [0] g0 of the current pipeline wave, [1] total paths
+    int32_t n;
+    int32_t cap;
 };
 
 } // namespace tpt

@@ -14,6 +14,7 @@
 #include <type_traits>
 
 #include "dev_scene.h"
+#include "nrc.h"
 
 namespace tpt {
 
@@ -595,15 +596,9 @@ __global__ TPT_TRAVERSAL_ATTR void k_trace_pt(DScene S, SrcRays src, uint32_t n,
 }
 
 // ---------------------------------------------------------------- generate
-__global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_paths, uint64_t base_mixed) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) {
-        P.counters[0] = (uint32_t)n_paths;
-        P.counters[12] = 0u; // extend fetch counter
-    }
-    if (j >= n_paths) return;
-    int pix = j % R.pixels;
-    int s = wave_base + j / R.pixels;
+// Camera ray and erand48 stream of sample `s` of pixel `pix`, written to slot j.
+__device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, const DPaths &P, int j, int pix, int s,
+                                         uint64_t base_mixed) {
     int W = S.cam.width;
     int x = pix % W, y = pix / W;
     uint64_t key = stream_seed(base_mixed, (uint64_t)pix, (uint64_t)s);
@@ -636,6 +631,16 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
     P.key[j] = key;
     P.depth[j] = 0;
     P.queue_a[j] = j;
+}
+
+__global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_paths, uint64_t base_mixed) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) {
+        P.counters[0] = (uint32_t)n_paths;
+        P.counters[12] = 0u; // extend fetch counter
+    }
+    if (j >= n_paths) return;
+    gen_path(S, R, P, j, j % R.pixels, wave_base + j / R.pixels, base_mixed);
 }
 
 // ---------------------------------------------------------------- extend
@@ -733,106 +738,161 @@ __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRe
     sh = SurfHit{n, sn, uv, into};
 }
 
-__global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel) {
+// One path vertex (the loop body of cpu/path_tracer.cpp:102-168) for the ray at
+// queue position i (slot `slot`): adds emission / background to the path's
+// radiance, samples the next direction and, for a non-specular lobe with delta
+// lights, prepares one shadow ray.  `alive`: the path continues.
+struct ShadeOut {
+    bool alive, emit_shadow;
+    V3 s_org, s_dir, s_val;
+    float s_tmax;
+    int cache_kind; // 0: none, 1: terminated into the radiance cache, 2: training record
+};
+
+// Roughness and albedo of a material's first lobe (radiance-cache inputs).
+__device__ __forceinline__ void lobe_summary(const DScene &S, const DMaterial &m, V2 uv, float &rough, V3 &alb) {
+    const DLobe &b = m.lobe[0];
+    rough = b.type == 0 ? 1.f : (b.type == 1 || b.type == 2) ? 0.f : sqrtf(b.alpha2);
+    alb = color_at(S, b.color, uv);
+}
+
+// One training path in `stride` per frame, by a hash of (frame, slot).
+__device__ __forceinline__ bool cache_trains(const DCache &C, int slot) {
+    return splitmix64(((uint64_t)C.frame << 32) ^ (uint64_t)(uint32_t)slot) % (uint64_t)C.train_stride == 0;
+}
+
+__device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &R, const DPaths &P, uint32_t i,
+                                                int slot, const DCache *C = nullptr) {
+    const float LIGHT_PROB = 1.f / 5.f; // cpu/path_tracer.cpp:104
+    ShadeOut out_{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
+    bool &alive = out_.alive, &emit_shadow = out_.emit_shadow;
+    V3 &s_org = out_.s_org, &s_dir = out_.s_dir, &s_val = out_.s_val;
+    float &s_tmax = out_.s_tmax;
+    float4 h4 = P.hit[i];
+    HitRec h{h4.x, __float_as_int(h4.y), h4.z, h4.w};
+    V3 o = mk4(P.org[slot]), d = mk4(P.dir[slot]);
+    V3 decay = mk4(P.decay[slot]);
+    V3 color = mk4(P.color[slot]);
+    int depth = P.depth[slot];
+    Rng rng{P.rng[slot]};
+    if (!(h.t < kINF)) {
+        color = add(color, cmul(decay, background(S, d)));
+    } else {
+        SurfHit sh;
+        int mi;
+        surface(S, o, d, h, sh, mi);
+        const DMaterial &m = S.materials[mi];
+        color = add(color, cmul(decay, color_at(S, m.emission, sh.uv)));
+        bool into_cache = false;
+        if (C && depth == C->depth) {
+            V3 hp = at(o, d, h.t);
+            V3 p01{(hp.x - C->lo[0]) * C->inv_ext[0], (hp.y - C->lo[1]) * C->inv_ext[1],
+                   (hp.z - C->lo[2]) * C->inv_ext[2]};
+            float rough;
+            V3 alb;
+            lobe_summary(S, m, sh.uv, rough, alb);
+            nrc_encode(C->feat + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal, rough, alb);
+            if (cache_trains(*C, slot)) {
+                out_.cache_kind = 2;
+                C->rec_T[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
+                C->rec_C[slot] = make_float4(color.x, color.y, color.z, 0.f);
+            } else {
+                out_.cache_kind = 1;
+                into_cache = true; // P.decay[slot] keeps the throughput arriving here
+            }
+        }
+        if (depth < R.max_depth && !into_cache) {
+            V3 hp = at(o, d, h.t);
+            float pdf_bxdf;
+            bool spec;
+            int sel = sample_bxdf(m, rng, pdf_bxdf, spec);
+            V3 out, albedo;
+            if (!spec) {
+                if (S.num_delta > 0) {
+                    const DDelta &dl = S.delta[rng.rand_int(S.num_delta)];
+                    float tmax;
+                    V3 tl = delta_sample(dl, hp, rng, tmax);
+                    if (!R.shadow_bounded) tmax = kINF;
+                    V3 la = bxdf_f(S, m, sel, d, sh, tl);
+                    float pdf_light = 1.f / (float)S.num_delta;
+                    V3 contrib = dvs(cmul(cmul(decay, la), mk(dl.emission)), pdf_light);
+                    if (contrib.x != 0.f || contrib.y != 0.f || contrib.z != 0.f) {
+                        emit_shadow = true;
+                        s_org = hp;
+                        s_dir = tl;
+                        s_val = contrib;
+                        s_tmax = tmax;
+                    }
+                }
+                float pdf_val;
+                if (S.num_lights > 0) {
+                    if (rng.random() < LIGHT_PROB) {
+                        out = lights_sample(S, hp, rng);
+                    } else {
+                        out = bxdf_sample(m, sel, d, sh, rng);
+                    }
+                    pdf_val = (1.f - LIGHT_PROB) * bxdf_pdf(m, sel, d, sh, out) +
+                              LIGHT_PROB * lights_pdf(S, hp, out);
+                } else {
+                    out = bxdf_sample(m, sel, d, sh, rng);
+                    pdf_val = bxdf_pdf(m, sel, d, sh, out);
+                }
+                albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / ((pdf_val + kEPS) * pdf_bxdf));
+            } else {
+                out = bxdf_sample(m, sel, d, sh, rng);
+                albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / pdf_bxdf);
+            }
+            decay = cmul(decay, albedo);
+            alive = !is_zero5(decay);
+            if (alive && R.rr_depth >= 0 && depth >= R.rr_depth) {
+                float mx = fmaxf(decay.x, fmaxf(decay.y, decay.z));
+                float q = fminf(mx, 0.95f);
+                if (rng.random() >= q) {
+                    alive = false;
+                } else {
+                    decay = dvs(decay, q);
+                }
+            }
+            if (alive) {
+                P.org[slot] = make_float4(hp.x, hp.y, hp.z, 0.f);
+                P.dir[slot] = make_float4(out.x, out.y, out.z, 0.f);
+                P.decay[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
+                P.depth[slot] = depth + 1;
+            }
+        }
+    }
+    P.color[slot] = make_float4(color.x, color.y, color.z, 0.f);
+    P.rng[slot] = rng.x;
+    return out_;
+}
+
+// CACHE: the radiance-cache hooks are on; queries and training records are
+// appended to C's lists (at most C.train_cap training records).
+template <bool CACHE>
+__global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel,
+                        DCache C) {
     uint32_t n = P.counters[qsel];
     if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[12] = 0u; // fetch counter of the next extend
-    const float LIGHT_PROB = 1.f / 5.f; // cpu/path_tracer.cpp:104
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
-        bool alive = false;
-        bool emit_shadow = false;
-        int slot = 0;
-        V3 s_org{0, 0, 0}, s_dir{0, 0, 0}, s_val{0, 0, 0};
-        float s_tmax = kINF;
-        if (valid) {
-            slot = queue[i];
-            float4 h4 = P.hit[i];
-            HitRec h{h4.x, __float_as_int(h4.y), h4.z, h4.w};
-            V3 o = mk4(P.org[slot]), d = mk4(P.dir[slot]);
-            V3 decay = mk4(P.decay[slot]);
-            V3 color = mk4(P.color[slot]);
-            int depth = P.depth[slot];
-            Rng rng{P.rng[slot]};
-            if (!(h.t < kINF)) {
-                color = add(color, cmul(decay, background(S, d)));
-            } else {
-                SurfHit sh;
-                int mi;
-                surface(S, o, d, h, sh, mi);
-                const DMaterial &m = S.materials[mi];
-                color = add(color, cmul(decay, color_at(S, m.emission, sh.uv)));
-                if (depth < R.max_depth) {
-                    V3 hp = at(o, d, h.t);
-                    float pdf_bxdf;
-                    bool spec;
-                    int sel = sample_bxdf(m, rng, pdf_bxdf, spec);
-                    V3 out, albedo;
-                    if (!spec) {
-                        if (S.num_delta > 0) {
-                            const DDelta &dl = S.delta[rng.rand_int(S.num_delta)];
-                            float tmax;
-                            V3 tl = delta_sample(dl, hp, rng, tmax);
-                            if (!R.shadow_bounded) tmax = kINF;
-                            V3 la = bxdf_f(S, m, sel, d, sh, tl);
-                            float pdf_light = 1.f / (float)S.num_delta;
-                            V3 contrib = dvs(cmul(cmul(decay, la), mk(dl.emission)), pdf_light);
-                            if (contrib.x != 0.f || contrib.y != 0.f || contrib.z != 0.f) {
-                                emit_shadow = true;
-                                s_org = hp;
-                                s_dir = tl;
-                                s_val = contrib;
-                                s_tmax = tmax;
-                            }
-                        }
-                        float pdf_val;
-                        if (S.num_lights > 0) {
-                            if (rng.random() < LIGHT_PROB) {
-                                out = lights_sample(S, hp, rng);
-                            } else {
-                                out = bxdf_sample(m, sel, d, sh, rng);
-                            }
-                            pdf_val = (1.f - LIGHT_PROB) * bxdf_pdf(m, sel, d, sh, out) +
-                                      LIGHT_PROB * lights_pdf(S, hp, out);
-                        } else {
-                            out = bxdf_sample(m, sel, d, sh, rng);
-                            pdf_val = bxdf_pdf(m, sel, d, sh, out);
-                        }
-                        albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / ((pdf_val + kEPS) * pdf_bxdf));
-                    } else {
-                        out = bxdf_sample(m, sel, d, sh, rng);
-                        albedo = scr(bxdf_f(S, m, sel, d, sh, out), 1.f / pdf_bxdf);
-                    }
-                    decay = cmul(decay, albedo);
-                    alive = !is_zero5(decay);
-                    if (alive && R.rr_depth >= 0 && depth >= R.rr_depth) {
-                        float mx = fmaxf(decay.x, fmaxf(decay.y, decay.z));
-                        float q = fminf(mx, 0.95f);
-                        if (rng.random() >= q) {
-                            alive = false;
-                        } else {
-                            decay = dvs(decay, q);
-                        }
-                    }
-                    if (alive) {
-                        P.org[slot] = make_float4(hp.x, hp.y, hp.z, 0.f);
-                        P.dir[slot] = make_float4(out.x, out.y, out.z, 0.f);
-                        P.decay[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
-                        P.depth[slot] = depth + 1;
-                    }
-                }
-            }
-            P.color[slot] = make_float4(color.x, color.y, color.z, 0.f);
-            P.rng[slot] = rng.x;
+        int slot = valid ? queue[i] : 0;
+        ShadeOut r{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
+        if (valid) r = shade_entry(S, R, P, i, slot, CACHE ? &C : nullptr);
+        if (CACHE) {
+            int qp = wave_append(r.cache_kind == 1, C.counters + 0);
+            if (r.cache_kind == 1) C.q_slot[qp] = slot;
+            int tp = wave_append(r.cache_kind == 2, C.counters + 1);
+            if (r.cache_kind == 2 && tp < C.train_cap) C.t_slot[tp] = slot;
         }
-        int pos = wave_append(alive, P.counters + nsel);
-        if (alive) next[pos] = slot;
-        int spos = wave_append(emit_shadow, P.counters + 2);
-        if (emit_shadow) {
+        int pos = wave_append(r.alive, P.counters + nsel);
+        if (r.alive) next[pos] = slot;
+        int spos = wave_append(r.emit_shadow, P.counters + 2);
+        if (r.emit_shadow) {
             P.shadow_slot[spos] = slot;
-            P.shadow_org[spos] = make_float4(s_org.x, s_org.y, s_org.z, s_tmax);
-            P.shadow_dir[spos] = make_float4(s_dir.x, s_dir.y, s_dir.z, 0.f);
-            P.shadow_val[spos] = make_float4(s_val.x, s_val.y, s_val.z, 0.f);
+            P.shadow_org[spos] = make_float4(r.s_org.x, r.s_org.y, r.s_org.z, r.s_tmax);
+            P.shadow_dir[spos] = make_float4(r.s_dir.x, r.s_dir.y, r.s_dir.z, 0.f);
+            P.shadow_val[spos] = make_float4(r.s_val.x, r.s_val.y, r.s_val.z, 0.f);
         }
     }
 }
@@ -857,20 +917,24 @@ __global__ void k_shadow(DScene S, DPaths P) {
 }
 
 // ---------------------------------------------------------------- accumulate / finalize
-__global__ void k_accumulate(DRender R, DPaths P, int wave_base, int k_active, int first_wave) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R.pixels) return;
+// Adds `count` consecutive samples (s_first, s_first + 1, ...) of one pixel to its
+// running sums at accumulator index q, in sample order; `color_slot(j)` is the
+// path slot of sample s_first + j.  A sub-pixel closes after nsub samples
+// (cpu/path_tracer.cpp:80-96): its mean is clipped (LDR) and added to the pixel.
+template <class SlotOf>
+__device__ __forceinline__ void accum_run(const DRender &R, const DPaths &P, size_t q, int s_first, int count,
+                                          SlotOf color_slot, bool first_wave) {
     float sf[3], pf[3];
     double sd[3], pd[3];
     for (int c = 0; c < 3; c++) {
-        sf[c] = first_wave ? 0.f : P.sub_f[3 * p + c];
-        pf[c] = first_wave ? 0.f : P.pix_f[3 * p + c];
-        sd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.sub_d[3 * p + c];
-        pd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.pix_d[3 * p + c];
+        sf[c] = first_wave ? 0.f : P.sub_f[3 * q + c];
+        pf[c] = first_wave ? 0.f : P.pix_f[3 * q + c];
+        sd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.sub_d[3 * q + c];
+        pd[c] = (first_wave || !R.accum_fp64) ? 0.0 : P.pix_d[3 * q + c];
     }
-    for (int j = 0; j < k_active; j++) {
-        int s = wave_base + j;
-        float4 L = P.color[(size_t)j * R.pixels + p];
+    for (int j = 0; j < count; j++) {
+        int s = s_first + j;
+        float4 L = P.color[color_slot(j)];
         if (R.accum_fp64) {
             sd[0] += L.x;
             sd[1] += L.y;
@@ -895,29 +959,35 @@ __global__ void k_accumulate(DRender R, DPaths P, int wave_base, int k_active, i
         }
     }
     for (int c = 0; c < 3; c++) {
-        P.sub_f[3 * p + c] = sf[c];
-        P.pix_f[3 * p + c] = pf[c];
+        P.sub_f[3 * q + c] = sf[c];
+        P.pix_f[3 * q + c] = pf[c];
         if (R.accum_fp64) {
-            P.sub_d[3 * p + c] = sd[c];
-            P.pix_d[3 * p + c] = pd[c];
+            P.sub_d[3 * q + c] = sd[c];
+            P.pix_d[3 * q + c] = pd[c];
         }
     }
+}
+
+__global__ void k_accumulate(DRender R, DPaths P, int wave_base, int k_active, int first_wave) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= R.pixels) return;
+    accum_run(R, P, (size_t)p, wave_base, k_active, [&](int j) { return (size_t)j * R.pixels + p; }, first_wave != 0);
+}
+
+// Output value of channel c at accumulator index q.
+__device__ __forceinline__ float final_value(const DRender &R, const DPaths &P, size_t q, int c) {
+    int bb = R.block * R.block;
+    if (R.output == 1) {
+        if (R.accum_fp64) return (float)(P.pix_d[3 * q + c] / ((double)R.nsub * (double)bb));
+        return P.pix_f[3 * q + c] / (float)(R.nsub * bb);
+    }
+    return clip1(P.pix_f[3 * q + c] / (float)bb);
 }
 
 __global__ void k_finalize(DRender R, DPaths P, float *out) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= R.pixels) return;
-    int bb = R.block * R.block;
-    for (int c = 0; c < 3; c++) {
-        float v;
-        if (R.output == 1) {
-            if (R.accum_fp64) v = (float)(P.pix_d[3 * p + c] / ((double)R.nsub * (double)bb));
-            else v = P.pix_f[3 * p + c] / (float)(R.nsub * bb);
-        } else {
-            v = clip1(P.pix_f[3 * p + c] / (float)bb);
-        }
-        out[3 * p + c] = v;
-    }
+    for (int c = 0; c < 3; c++) out[3 * p + c] = final_value(R, P, (size_t)p, c);
 }
 
 // ---------------------------------------------------------------- ray queries (parity / bench)
@@ -950,6 +1020,254 @@ __global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8
         HitRec h = trace<true, false, NL, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
         occ[i] = h.gid >= 0 ? 1 : 0;
     }
+}
+
+// ---------------------------------------------------------------- radiance cache (progressive frames)
+__global__ void k_cache_reset(DCache C) {
+    if (threadIdx.x < 2) C.counters[threadIdx.x] = 0u;
+}
+
+// color += decay * max(prediction, 0) for every path that terminated into the cache
+__global__ void k_cache_apply(DPaths P, DCache C, const float4 *pred, int n_max) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = min((int)C.counters[0], n_max);
+    if (i >= n) return;
+    int s = C.q_slot[i];
+    float4 p = pred[i], d = P.decay[s], c = P.color[s];
+    P.color[s] = make_float4(c.x + d.x * fmaxf(p.x, 0.f), c.y + d.y * fmaxf(p.y, 0.f), c.z + d.z * fmaxf(p.z, 0.f), 0.f);
+}
+
+// Training targets: the radiance each recorded vertex scattered towards the
+// camera, (final radiance - radiance after the vertex's emission) / throughput.
+__device__ __forceinline__ float cache_target(float fin, float at, float T) {
+    if (!(T > 1e-6f)) return 0.f;
+    return fminf(fmaxf((fin - at) / T, 0.f), 1e4f);
+}
+__global__ void k_cache_targets(DPaths P, DCache C, float4 *target) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = min((int)C.counters[1], C.train_cap);
+    if (i >= n) return;
+    int s = C.t_slot[i];
+    float4 f = P.color[s], a = C.rec_C[s], T = C.rec_T[s];
+    target[i] = make_float4(cache_target(f.x, a.x, T.x), cache_target(f.y, a.y, T.y), cache_target(f.z, a.z, T.z), 0.f);
+}
+
+// One-sample-per-pixel frame (slot = pixel) added to the running per-pixel sums.
+__global__ void k_frame_accumulate(DPaths P, int pixels, int fp64, int first) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float4 L = P.color[p];
+    const float v[3] = {L.x, L.y, L.z};
+    for (int c = 0; c < 3; c++) {
+        if (fp64) P.pix_d[3 * p + c] = (first ? 0.0 : P.pix_d[3 * p + c]) + (double)v[c];
+        else P.pix_f[3 * p + c] = (first ? 0.f : P.pix_f[3 * p + c]) + v[c];
+    }
+}
+__global__ void k_frame_finalize(DPaths P, int pixels, int fp64, int frames, float *out) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    for (int c = 0; c < 3; c++)
+        out[3 * p + c] = fp64 ? (float)(P.pix_d[3 * p + c] / (double)frames) : P.pix_f[3 * p + c] / (float)frames;
+}
+
+// ---------------------------------------------------------------- batch mode
+// Many independent scenes share one path pool: the kernels below are the batch
+// counterparts of generate / extend / shade / shadow / accumulate, built on the
+// same per-path device functions, so every scene's image equals its
+// single-scene render bit for bit.
+#define TPT_SEG(B, f, s) (B).seg[(f) * (B).n + (s)]
+
+// Largest s in [0, n) with off[s] <= v (off non-decreasing, off[0] <= v).
+template <class T> __device__ __forceinline__ int find_seg(const T *off, int n, T v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t batch_wave_paths(const DBatch &B) {
+    uint64_t g0 = B.state[0], total = B.state[1];
+    return total > g0 ? min((uint64_t)B.cap, total - g0) : 0;
+}
+
+// Ray-queue segments of the pipeline wave starting at g0: scene s owns slots
+// [path_off[s] - g0, path_off[s + 1] - g0) clipped to the wave.
+__global__ void k_batch_begin(DBatch B) {
+    uint64_t g0 = B.state[0], nw = batch_wave_paths(B);
+    for (int s = threadIdx.x; s < B.n; s += blockDim.x) {
+        uint64_t lo = B.path_off[s] > g0 ? min(B.path_off[s] - g0, nw) : 0;
+        uint64_t hi = B.path_off[s + 1] > g0 ? min(B.path_off[s + 1] - g0, nw) : 0;
+        TPT_SEG(B, kSegQaBase, s) = (int32_t)lo;
+        TPT_SEG(B, kSegQaCnt, s) = (int32_t)(hi - lo);
+        TPT_SEG(B, kSegFetch, s) = 0;
+    }
+}
+
+__global__ void k_generate_batch(DBatch B, DPaths P) {
+    uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (j >= batch_wave_paths(B)) return;
+    uint64_t g = B.state[0] + j;
+    int s = find_seg(B.path_off, B.n, g);
+    const DRender &R = B.renders[s];
+    uint64_t l = g - B.path_off[s];
+    gen_path(B.scenes[s], R, P, (int)j, (int)(l % (uint64_t)R.pixels), (int)(l / (uint64_t)R.pixels), B.seed_mixed[s]);
+    P.sid[j] = s;
+}
+
+// Persistent traversal over the per-scene segments: each wavefront starts at its
+// own scene (spreading the wavefronts over the batch) and walks all scenes,
+// draining one scene's segment before moving to the next, so the scene is
+// uniform across the wavefront and its record is read through scalar loads.
+__device__ __forceinline__ int batch_start_scene(const DBatch &B) {
+    int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    return (int)(((int64_t)wave * B.n) / nwaves);
+}
+
+template <int NL>
+__global__ TPT_TRAVERSAL_ATTR void k_extend_batch(DBatch B, DPaths P, int qsel) {
+    const int fb = qsel ? kSegQbBase : kSegQaBase;
+    const int32_t *queue = qsel ? P.queue_b : P.queue_a;
+    const int start = batch_start_scene(B);
+    uint32_t nodes = 0, tris = 0;
+    for (int k = 0; k < B.n; k++) {
+        int s = start + k < B.n ? start + k : start + k - B.n;
+        s = __builtin_amdgcn_readfirstlane(s);
+        int cnt = TPT_SEG(B, fb + 1, s);
+        if (cnt == 0) continue;
+        int base = TPT_SEG(B, fb, s);
+        SrcQueue src{queue + base, P.org, P.dir, P.key, P.depth, P.hit + base};
+        persistent_trace<false, false, NL, true, false>(B.scenes[s], src, (uint32_t)cnt,
+                                                        (uint32_t *)&TPT_SEG(B, kSegFetch, s), nodes, tris);
+    }
+}
+
+// Next-bounce segments: the survivors (and shadow rays) of scene s go to
+// [scan_s, scan_s + count_s) where scan is the exclusive prefix sum of this
+// bounce's per-scene ray counts (a path leaves at most one survivor and one
+// shadow ray).  One workgroup of 1024 threads, at most 1024 scenes.
+__global__ void k_batch_seg(DBatch B, int qsel) {
+    __shared__ int32_t buf[1024];
+    const int cc = (qsel ? kSegQbBase : kSegQaBase) + 1;
+    const int nb = qsel ? kSegQaBase : kSegQbBase;
+    int t = threadIdx.x;
+    int c = t < B.n ? TPT_SEG(B, cc, t) : 0;
+    buf[t] = c;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int v = t >= off ? buf[t - off] : 0;
+        __syncthreads();
+        buf[t] += v;
+        __syncthreads();
+    }
+    if (t < B.n) {
+        int excl = buf[t] - c;
+        TPT_SEG(B, nb, t) = excl;
+        TPT_SEG(B, nb + 1, t) = 0;
+        TPT_SEG(B, kSegShBase, t) = excl;
+        TPT_SEG(B, kSegShCnt, t) = 0;
+        TPT_SEG(B, kSegShFetch, t) = 0;
+        TPT_SEG(B, kSegFetch, t) = 0;
+    }
+}
+
+__global__ void k_shade_batch(DBatch B, DPaths P, int qsel) {
+    const int cb = qsel ? kSegQbBase : kSegQaBase, nb = qsel ? kSegQaBase : kSegQbBase;
+    const int32_t *queue = qsel ? P.queue_b : P.queue_a;
+    int32_t *next = qsel ? P.queue_a : P.queue_b;
+    const int32_t *cbase = B.seg + cb * B.n;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < (uint32_t)B.cap; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        int s = 0;
+        bool valid = false;
+        if (i < (uint32_t)B.cap) {
+            s = find_seg(cbase, B.n, (int32_t)i);
+            valid = (int32_t)i < cbase[s] + TPT_SEG(B, cb + 1, s);
+        }
+        int slot = valid ? queue[i] : 0;
+        // one pass per distinct scene among the wavefront's entries (usually one)
+        uint64_t todo = __ballot(valid);
+        while (todo) {
+            int lead = __ffsll((unsigned long long)todo) - 1;
+            int s0 = __builtin_amdgcn_readlane(s, lead);
+            bool mine = valid && s == s0;
+            ShadeOut r{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
+            if (mine) r = shade_entry(B.scenes[s0], B.renders[s0], P, i, slot);
+            int pos = wave_append(r.alive, (uint32_t *)&TPT_SEG(B, nb + 1, s0));
+            if (r.alive) next[TPT_SEG(B, nb, s0) + pos] = slot;
+            int spos = wave_append(r.emit_shadow, (uint32_t *)&TPT_SEG(B, kSegShCnt, s0));
+            if (r.emit_shadow) {
+                int k = TPT_SEG(B, kSegShBase, s0) + spos;
+                P.shadow_slot[k] = slot;
+                P.shadow_org[k] = make_float4(r.s_org.x, r.s_org.y, r.s_org.z, r.s_tmax);
+                P.shadow_dir[k] = make_float4(r.s_dir.x, r.s_dir.y, r.s_dir.z, 0.f);
+                P.shadow_val[k] = make_float4(r.s_val.x, r.s_val.y, r.s_val.z, 0.f);
+            }
+            todo &= ~__ballot(mine);
+        }
+    }
+}
+
+template <int NL>
+__global__ TPT_TRAVERSAL_ATTR void k_shadow_batch(DBatch B, DPaths P) {
+    const int start = batch_start_scene(B);
+    uint32_t nodes = 0, tris = 0;
+    for (int k = 0; k < B.n; k++) {
+        int s = start + k < B.n ? start + k : start + k - B.n;
+        s = __builtin_amdgcn_readfirstlane(s);
+        int cnt = TPT_SEG(B, kSegShCnt, s);
+        if (cnt == 0) continue;
+        int base = TPT_SEG(B, kSegShBase, s);
+        SrcShadow src{P.shadow_org + base, P.shadow_dir + base, P.shadow_val + base, P.shadow_slot + base, P.color};
+        persistent_trace<true, false, NL, true, false>(B.scenes[s], src, (uint32_t)cnt,
+                                                       (uint32_t *)&TPT_SEG(B, kSegShFetch, s), nodes, tris);
+    }
+}
+
+// One thread per pixel of every scene: adds the samples of this pipeline wave
+// that belong to the pixel, in sample order.
+__global__ void k_accumulate_batch(DBatch B, DPaths P) {
+    uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= B.pix_off[B.n]) return;
+    int s = find_seg(B.pix_off, B.n, q);
+    const DRender &R = B.renders[s];
+    uint64_t p = q - B.pix_off[s];
+    uint64_t g0 = B.state[0], nw = batch_wave_paths(B);
+    uint64_t a = max(B.path_off[s], g0), b = min(B.path_off[s + 1], g0 + nw);
+    if (a >= b) return;
+    a -= B.path_off[s];
+    b -= B.path_off[s];
+    uint64_t px = (uint64_t)R.pixels;
+    uint64_t s_lo = a > p ? (a - p + px - 1) / px : 0;
+    if (s_lo * px + p >= b) return;
+    uint64_t s_hi = (b - 1 - p) / px;
+    uint64_t slot0 = B.path_off[s] - g0 + p; // slot of sample 0 of this pixel (may wrap below 0)
+    accum_run(R, P, q, (int)s_lo, (int)(s_hi - s_lo + 1), [&](int j) { return slot0 + (s_lo + j) * px; }, false);
+}
+
+__global__ void k_batch_end(DBatch B) {
+    if (threadIdx.x == 0) B.state[0] += (uint64_t)B.cap;
+}
+
+__global__ void k_batch_zero(DBatch B, DPaths P, int fp64) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= 3 * B.pix_off[B.n]) return;
+    P.sub_f[i] = 0.f;
+    P.pix_f[i] = 0.f;
+    if (fp64) {
+        P.sub_d[i] = 0.0;
+        P.pix_d[i] = 0.0;
+    }
+}
+
+__global__ void k_finalize_batch(DBatch B, DPaths P, float *out) {
+    uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (q >= B.pix_off[B.n]) return;
+    int s = find_seg(B.pix_off, B.n, q);
+    for (int c = 0; c < 3; c++) out[3 * q + c] = final_value(B.renders[s], P, q, c);
 }
 
 // ---------------------------------------------------------------- host-visible launchers
@@ -1014,8 +1332,11 @@ void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int q
     });
 }
 void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int32_t *queue, int32_t *next, int qsel,
-                  int nsel, int grid, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel);
+                  int nsel, int grid, int block, hipStream_t st, const DCache *C) {
+    if (C)
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel, *C);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel, DCache{});
 }
 void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
     by_layout(S.layout, [&](auto L) {
@@ -1085,6 +1406,57 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
                                (uint32_t)n, counters);
         });
     });
+}
+
+// One pipeline wave of a batch (global paths [g0, g0 + cap), g0 read on the
+// device): every launch has a fixed shape and all wave-dependent values live in
+// device memory, so the sequence can be captured once in a hipGraph and replayed
+// ceil(total / cap) times.
+void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layout, int block, int trav_block,
+                       int num_cu, uint64_t total_pixels, hipStream_t st) {
+    auto cu_grid = [&](const void *k, int tb) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, tb, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        return num_cu * per_cu;
+    };
+    int gen_grid = (B.cap + block - 1) / block;
+    int shade_grid = std::min(gen_grid, num_cu * 2048 / block);
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
+    by_layout(layout, [&](auto L) {
+        constexpr int NL = decltype(L)::value;
+        int eg = cu_grid((const void *)(k_extend_batch<NL>), trav_block);
+        int sg = cu_grid((const void *)(k_shadow_batch<NL>), trav_block);
+        for (int b = 0; b <= max_depth; b++) {
+            int qsel = b & 1;
+            hipLaunchKernelGGL((k_extend_batch<NL>), dim3(eg), dim3(trav_block), 0, st, B, P, qsel);
+            hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+            hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
+            hipLaunchKernelGGL((k_shadow_batch<NL>), dim3(sg), dim3(trav_block), 0, st, B, P);
+        }
+    });
+    hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
+    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
+}
+void launch_batch_zero(const DBatch &B, const DPaths &P, int fp64, uint64_t total_pixels, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_zero, dim3((unsigned)((3 * total_pixels + 255) / 256)), dim3(256), 0, st, B, P, fp64);
+}
+void launch_batch_finalize(const DBatch &B, const DPaths &P, float *out, uint64_t total_pixels, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P, out);
+}
+
+void launch_cache_reset(const DCache &C, hipStream_t st) { hipLaunchKernelGGL(k_cache_reset, dim3(1), dim3(64), 0, st, C); }
+void launch_cache_apply(const DPaths &P, const DCache &C, const float4 *pred, int n_max, hipStream_t st) {
+    hipLaunchKernelGGL(k_cache_apply, dim3((n_max + 255) / 256), dim3(256), 0, st, P, C, pred, n_max);
+}
+void launch_cache_targets(const DPaths &P, const DCache &C, float4 *target, hipStream_t st) {
+    hipLaunchKernelGGL(k_cache_targets, dim3((C.train_cap + 255) / 256), dim3(256), 0, st, P, C, target);
+}
+void launch_frame_accumulate(const DPaths &P, int pixels, int fp64, int first, hipStream_t st) {
+    hipLaunchKernelGGL(k_frame_accumulate, dim3((pixels + 255) / 256), dim3(256), 0, st, P, pixels, fp64, first);
+}
+void launch_frame_finalize(const DPaths &P, int pixels, int fp64, int frames, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_frame_finalize, dim3((pixels + 255) / 256), dim3(256), 0, st, P, pixels, fp64, frames, out);
 }
 
 } // namespace tpt

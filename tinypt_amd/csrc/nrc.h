@@ -1,0 +1,78 @@
+// nrc.h — learned radiance cache: weight storage, the 64-dimensional input
+// encoding used by the shade kernel, and the MLP kernel launchers (nrc.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dev_math.h"
+
+namespace tpt {
+
+// FP32 -> BF16 bits, round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Device buffers of the MLP: 5 layers x 64 x 64 (the last layer uses 3 outputs).
+struct NrcWeights {
+    float *w;      // FP32 master weights [l][in][out]
+    float *m, *v;  // Adam moments
+    float *grad;   // gradient accumulator (zeroed by the Adam kernel)
+    uint16_t *wt;  // BF16 [l][out][in] (forward)
+    uint16_t *wb;  // BF16 [l][in][out] (backward)
+    float *loss;   // running sum of the per-step mean loss
+};
+static constexpr int kNrcParams = 5 * 64 * 64;
+static constexpr int kNrcInputs = 64;
+
+// Input encoding (64 values): position normalised to the scene bounds with
+// sin / cos at 6 octaves per axis (36), the outgoing direction and the shading
+// normal at 2 octaves per axis (12 + 12), roughness (1) and albedo (3).
+__device__ inline void nrc_encode(uint16_t *dst, V3 pos01, V3 wo, V3 n, float rough, V3 albedo) {
+    uint16_t f[64];
+    int k = 0;
+    const float pv[3] = {pos01.x, pos01.y, pos01.z};
+    for (int a = 0; a < 3; a++) {
+        float x = pv[a] * 3.14159265f;
+        for (int o = 0; o < 6; o++) {
+            float s, c;
+            __sincosf(x, &s, &c);
+            f[k++] = f32_to_bf16_bits(s);
+            f[k++] = f32_to_bf16_bits(c);
+            x *= 2.f;
+        }
+    }
+    const float dv[6] = {wo.x, wo.y, wo.z, n.x, n.y, n.z};
+    for (int a = 0; a < 6; a++) {
+        float x = dv[a] * 3.14159265f;
+        for (int o = 0; o < 2; o++) {
+            float s, c;
+            __sincosf(x, &s, &c);
+            f[k++] = f32_to_bf16_bits(s);
+            f[k++] = f32_to_bf16_bits(c);
+            x *= 2.f;
+        }
+    }
+    f[60] = f32_to_bf16_bits(rough);
+    f[61] = f32_to_bf16_bits(albedo.x);
+    f[62] = f32_to_bf16_bits(albedo.y);
+    f[63] = f32_to_bf16_bits(albedo.z);
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+    const uint4 *f4 = reinterpret_cast<const uint4 *>(f);
+    for (int i = 0; i < 8; i++) d4[i] = f4[i];
+}
+
+// feats: records of 64 BF16; `idx` (may be null) maps record i to row idx[i] of
+// feats; the record count is read from *n_dev (at most n_max).
+void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t *n_dev, int n_max,
+                      const uint16_t *wt, float4 *out, hipStream_t st);
+// Trains on records [base, base + n_max) (clamped to the count at *n_dev).
+void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
+                      int base, int n_max, const NrcWeights &W, hipStream_t st);
+void nrc_launch_adam(const NrcWeights &W, float lr, int step, hipStream_t st);
+
+} // namespace tpt

@@ -1,0 +1,298 @@
+// nrc.hip — the learned radiance cache: a 5-layer, 64-wide ReLU MLP (no biases)
+// evaluated and trained with gfx950 BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulate).  No counterpart in the reference renderer; it terminates paths of
+// the wavefront tracer into the cache after a fixed number of bounces.
+//
+// Layers: 64 -> 64 -> 64 -> 64 -> 64 -> 3 (the last layer is stored padded to 64
+// outputs; its padding columns stay zero).  Parameters are kept as FP32 master
+// weights W[l][in][out] (Adam state beside them) plus two BF16 copies:
+//   wt[l][out][in]  forward operand  (B[k = in][n = out], 8 consecutive k per lane)
+//   wb[l][in][out]  backward operand (B[k = out][n = in], 8 consecutive k per lane)
+// so every MFMA B operand is one 16-byte load.
+//
+// MFMA 16x16x32 lane layout (wave64, lane l, j = 0..7 / 0..3):
+//   A[m = l % 16][k = (l / 16) * 8 + j]     B[k = (l / 16) * 8 + j][n = l % 16]
+//   D[m = (l / 16) * 4 + j][n = l % 16]
+// A wavefront processes a tile of 64 samples: M = 4 tiles of 16 samples,
+// N = 4 tiles of 16 features, K = 2 steps of 32 features.  Activations of the
+// tile are staged in LDS as [sample][feature] BF16 rows (stride kRow).
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "nrc.h"
+
+namespace tpt {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+static constexpr int kRow = 72; // LDS row stride in BF16 elements (64 + 8: rows start on different banks)
+
+__device__ __forceinline__ uint16_t f2bf(float f) { return f32_to_bf16_bits(f); }
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// 8 consecutive BF16 values as an MFMA operand
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+__device__ __forceinline__ bf16x8 ld8_lds(const uint16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+// 8 BF16 values with stride `stride` elements (transposed operand gathers)
+__device__ __forceinline__ bf16x8 gather8(const uint16_t *p, int stride) {
+    uint16_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = p[j * stride];
+    bf16x8 r;
+    __builtin_memcpy(&r, v, 16);
+    return r;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// One forward layer for the wavefront's 64-sample tile: out_lds = relu(in_lds . W)
+// (hidden layers) or, for the last layer, the three outputs in `pred`.
+template <bool LAST>
+__device__ __forceinline__ void layer_fwd(const uint16_t *in_lds, uint16_t *out_lds, const uint16_t *wt,
+                                          float pred[4][4]) {
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, q = l >> 4;
+    constexpr int NT = LAST ? 1 : 4;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) b[nt] = ld8(wt + (nt * 16 + r) * 64 + ks * 32 + q * 8);
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            bf16x8 a = ld8_lds(in_lds + (mt * 16 + r) * kRow + ks * 32 + q * 8);
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma(a, b[nt], acc[mt][nt]);
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float v = acc[mt][nt][j];
+                if (LAST) pred[mt][j] = v; // sample mt*16 + q*4 + j, output n = r (valid for r < 3)
+                else out_lds[(mt * 16 + q * 4 + j) * kRow + nt * 16 + r] = f2bf(fmaxf(v, 0.f));
+            }
+}
+
+// ---------------------------------------------------------------- inference
+// Record count (clamped to n_max) and the feature row of record i.
+__device__ __forceinline__ int rec_count(const uint32_t *n_dev, int n_max) { return min((int)*n_dev, n_max); }
+__device__ __forceinline__ const uint16_t *rec_row(const uint16_t *feats, const int32_t *idx, int i) {
+    return feats + (size_t)(idx ? idx[i] : i) * 64;
+}
+// Loads the feature rows of records [tile0, tile0 + 64) into an LDS tile (zeros beyond n).
+__device__ __forceinline__ void load_tile(uint16_t *dst, const uint16_t *feats, const int32_t *idx, int tile0, int n) {
+    const int l = threadIdx.x & 63;
+    for (int i = l; i < 64 * 8; i += 64) {
+        int s = i >> 3, c = (i & 7) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (tile0 + s < n) v = *reinterpret_cast<const uint4 *>(rec_row(feats, idx, tile0 + s) + c);
+        *reinterpret_cast<uint4 *>(dst + s * kRow + c) = v;
+    }
+}
+
+// out[i]: predicted radiance (rgb, 0) of record i.
+__global__ void k_nrc_infer(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
+                            const uint32_t *__restrict__ n_dev, int n_max, const uint16_t *__restrict__ wt,
+                            float4 *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[2][4][64 * kRow];
+    const int n = rec_count(n_dev, n_max);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int tile0 = (blockIdx.x * (blockDim.x >> 6) + w) * 64;
+    if (blockIdx.x * (blockDim.x >> 6) * 64 >= n) return; // whole workgroup beyond the records
+    uint16_t *a0 = act[0][w], *a1 = act[1][w];
+    load_tile(a0, feats, idx, tile0, n);
+    __syncthreads();
+    float pred[4][4];
+    layer_fwd<false>(a0, a1, wt + 0 * 4096, pred);
+    __syncthreads();
+    layer_fwd<false>(a1, a0, wt + 1 * 4096, pred);
+    __syncthreads();
+    layer_fwd<false>(a0, a1, wt + 2 * 4096, pred);
+    __syncthreads();
+    layer_fwd<false>(a1, a0, wt + 3 * 4096, pred);
+    __syncthreads();
+    layer_fwd<true>(a0, a1, wt + 4 * 4096, pred);
+    // lanes with output column r < 3 hold the predictions; gather them through LDS
+    float *o = reinterpret_cast<float *>(a1); // 64 x 4 floats (reuses a1)
+    __syncthreads();
+    const int r = l & 15, q = l >> 4;
+    if (r < 3)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) o[(mt * 16 + q * 4 + j) * 4 + r] = pred[mt][j];
+    __syncthreads();
+    if (tile0 + l < n) out[tile0 + l] = make_float4(o[l * 4], o[l * 4 + 1], o[l * 4 + 2], 0.f);
+}
+
+// ---------------------------------------------------------------- training
+// Forward + backward over one 64-sample tile per wavefront (one wavefront per
+// workgroup: the tile's five activation matrices and two gradient matrices fill
+// ~64 KB of LDS).  Loss: relative L2, sum_c (p_c - t_c)^2 / (sg(p_c)^2 + 0.01),
+// averaged over the n samples.  Weight gradients are summed into grad (FP32,
+// W[l][in][out] layout) with atomics; the mean loss is added to *loss.
+__global__ __launch_bounds__(64) void k_nrc_train(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
+                                                  const float4 *__restrict__ target, const uint32_t *__restrict__ n_dev,
+                                                  int base, int n_max, const uint16_t *__restrict__ wt,
+                                                  const uint16_t *__restrict__ wb, float *__restrict__ grad,
+                                                  float *__restrict__ loss) {
+    __shared__ __attribute__((aligned(16))) uint16_t act[5][64 * kRow]; // layer inputs x0..x4
+    __shared__ __attribute__((aligned(16))) uint16_t g[2][64 * kRow];   // output gradients (ping-pong)
+    const int n = min(max((int)*n_dev - base, 0), n_max); // records base .. base + n - 1
+    if (idx) idx += base;
+    target += base;
+    const int l = threadIdx.x;
+    const int r = l & 15, q = l >> 4;
+    const int tile0 = blockIdx.x * 64;
+    if (tile0 >= n) return;
+    load_tile(act[0], feats, idx, tile0, n);
+    __syncthreads();
+    float pred[4][4];
+    for (int ly = 0; ly < 4; ly++) {
+        layer_fwd<false>(act[ly], act[ly + 1], wt + ly * 4096, pred);
+        __syncthreads();
+    }
+    layer_fwd<true>(act[4], nullptr, wt + 4 * 4096, pred);
+    // loss gradient of the output (columns 0..2), written as BF16 into g[0][s][0..31]
+    // (columns 3..31 zero: the last layer's K step for the backward pass is 32)
+    for (int i = l; i < 64 * 32; i += 64) g[0][(i >> 5) * kRow + (i & 31)] = 0;
+    __syncthreads();
+    float lsum = 0.f;
+    const float inv_n = 1.f / (float)n;
+    if (r < 3) {
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                int s = mt * 16 + q * 4 + j;
+                float gp = 0.f;
+                if (tile0 + s < n) {
+                    float4 t4 = target[tile0 + s];
+                    float t = r == 0 ? t4.x : (r == 1 ? t4.y : t4.z);
+                    float p = pred[mt][j];
+                    float den = p * p + 0.01f;
+                    float d = p - t;
+                    lsum += d * d / den * inv_n;
+                    gp = 2.f * d / den * inv_n;
+                }
+                g[0][s * kRow + r] = f2bf(gp);
+            }
+    }
+    // wave-reduce the loss
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
+    if (l == 0) atomicAdd(loss, lsum);
+    __syncthreads();
+    // backward, last layer first: gout = g[cur] is dL/d(output of layer ly)
+    int cur = 0;
+    for (int ly = 4; ly >= 0; ly--) {
+        const uint16_t *x = act[ly];
+        const uint16_t *go = g[cur];
+        const int NO = ly == 4 ? 1 : 4; // output tiles of 16 (the last layer's 3 outputs sit in tile 0)
+        const int KO = ly == 4 ? 1 : 2; // K steps of 32 over the outputs
+        // dW[in][out] = sum_s x[s][in] * go[s][out]   (M = in, N = out, K = samples)
+        for (int mt = 0; mt < 4; mt++) {
+            f32x4 acc[4];
+            for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                bf16x8 a = gather8(x + (ks * 32 + q * 8) * kRow + mt * 16 + r, kRow);
+                for (int nt = 0; nt < NO; nt++) {
+                    bf16x8 b = gather8(go + (ks * 32 + q * 8) * kRow + nt * 16 + r, kRow);
+                    acc[nt] = mfma(a, b, acc[nt]);
+                }
+            }
+            for (int nt = 0; nt < NO; nt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int in = mt * 16 + q * 4 + j, out = nt * 16 + r;
+                    if (ly == 4 && out >= 3) continue;
+                    atomicAdd(grad + ly * 4096 + in * 64 + out, acc[nt][j]);
+                }
+        }
+        if (ly == 0) break;
+        // gin[s][in] = (sum_o go[s][o] * W[in][o]) * (x[s][in] > 0)   (M = samples, N = in, K = out)
+        uint16_t *gi = g[cur ^ 1];
+        const uint16_t *wl = wb + ly * 4096;
+        for (int mt = 0; mt < 4; mt++) {
+            f32x4 acc[4];
+            for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < KO; ks++) {
+                bf16x8 a = ld8_lds(go + (mt * 16 + r) * kRow + ks * 32 + q * 8);
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++) {
+                    bf16x8 b = ld8(wl + (nt * 16 + r) * 64 + ks * 32 + q * 8);
+                    acc[nt] = mfma(a, b, acc[nt]);
+                }
+            }
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    int s = mt * 16 + q * 4 + j, in = nt * 16 + r;
+                    float relu_on = bf2f(x[s * kRow + in]) > 0.f ? 1.f : 0.f;
+                    gi[s * kRow + in] = f2bf(acc[nt][j] * relu_on);
+                }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights, then the two BF16 copies.
+// grad is read and zeroed.  t: step number (1-based).
+__global__ void k_nrc_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                           float *__restrict__ grad, uint16_t *__restrict__ wt, uint16_t *__restrict__ wb, float lr,
+                           float b1, float b2, float eps, float bc1, float bc2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 5 * 4096) return;
+    int ly = i >> 12, in = (i >> 6) & 63, out = i & 63;
+    float gr = grad[i];
+    grad[i] = 0.f;
+    float mi = b1 * m[i] + (1.f - b1) * gr;
+    float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+    m[i] = mi;
+    v[i] = vi;
+    float wi = w[i];
+    if (ly < 4 || out < 3) wi -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    else wi = 0.f;
+    w[i] = wi;
+    uint16_t b = f2bf(wi);
+    wb[i] = b;                                // [l][in][out]
+    wt[(ly << 12) + out * 64 + in] = b;       // [l][out][in]
+}
+
+// ---------------------------------------------------------------- launchers
+void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t *n_dev, int n_max,
+                      const uint16_t *wt, float4 *out, hipStream_t st) {
+    if (n_max <= 0) return;
+    int tiles = (n_max + 63) / 64;
+    int blocks = (tiles + 3) / 4;
+    hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
+}
+void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
+                      int base, int n_max, const NrcWeights &W, hipStream_t st) {
+    if (n_max <= 0) return;
+    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64), 0, st, feats, idx, target, n_dev, base, n_max,
+                       W.wt, W.wb, W.grad, W.loss);
+}
+void nrc_launch_adam(const NrcWeights &W, float lr, int step, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_nrc_adam, dim3((5 * 4096 + 255) / 256), dim3(256), 0, st, W.w, W.m, W.v, W.grad, W.wt, W.wb,
+                       lr, b1, b2, eps, bc1, bc2);
+}
+
+} // namespace tpt

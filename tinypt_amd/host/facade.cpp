@@ -396,4 +396,41 @@ Image PathTracer::render(const Scene &scene, int num_samples, const RenderOption
     return img;
 }
 
+std::vector<Image> PathTracer::render_batch(const std::vector<Scene> &scenes, const std::vector<int> &num_samples,
+                                            const std::vector<uint64_t> &seeds, const RenderOptions &o) const {
+    TINYPT_CHECK_EX(!scenes.empty(), std::invalid_argument, "empty batch");
+    TINYPT_CHECK_EX(num_samples.size() == scenes.size() && seeds.size() == scenes.size(), std::invalid_argument,
+                    "expect one sample count and one seed per scene");
+    std::vector<tpt_hip_scene *> handles;
+    std::vector<tpt_batch_item> items;
+    std::vector<Image> imgs;
+    std::vector<float *> outs;
+    imgs.reserve(scenes.size());
+    for (size_t i = 0; i < scenes.size(); i++) {
+        TINYPT_CHECK_EX(scenes[i].is_hip(), std::runtime_error,
+                        "Expect scene on hip, got " + scenes[i].device().str() + " scene");
+        TINYPT_CHECK_EX(num_samples[i] > 0, std::invalid_argument, "num_samples must be positive");
+        handles.push_back(scenes[i].hip_handle());
+        tpt_batch_item it{};
+        it.spp = num_samples[i];
+        it.seed = seeds[i];
+        items.push_back(it);
+        imgs.emplace_back(scenes[i].camera().width, scenes[i].camera().height, 3);
+    }
+    for (auto &im : imgs) outs.push_back(im.f32().data());
+    tpt_render_params p{};
+    p.max_depth = o.max_depth;
+    p.rr_depth = o.rr_depth;
+    p.shadow_bounded = o.shadow_bounded ? 1 : 0;
+    p.output = o.hdr ? TPT_OUT_HDR_MEAN : TPT_OUT_LDR_CLIPPED;
+    p.accum_fp64 = o.fp64 ? 1 : 0;
+    p.queue_capacity = o.queue_capacity;
+    p.block_size = o.block_size;
+    _batch_scene_ms.assign(scenes.size(), 0.0);
+    int rc = tpt_hip_render_batch(handles.data(), items.data(), (int32_t)handles.size(), &p, outs.data(), &_batch_stats,
+                                  _batch_scene_ms.data());
+    if (rc != TPT_OK) throw_code(rc, __FILE__, __LINE__);
+    return imgs;
+}
+
 } // namespace tinypt

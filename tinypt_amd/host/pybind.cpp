@@ -163,7 +163,38 @@ PYBIND11_MODULE(_C, m) {
             py::arg("scene"), py::arg("num_samples"), py::arg("max_depth") = 5, py::arg("rr_depth") = -1,
             py::arg("shadow_bounded") = false, py::arg("seed") = 0, py::arg("hdr") = false, py::arg("fp64") = false,
             py::arg("queue_capacity") = 0, py::arg("block_size") = 0, py::arg("count_traffic") = false)
-        .def("last_stats", [](const PathTracer &self) { return stats_dict(self.last_stats()); });
+        .def("last_stats", [](const PathTracer &self) { return stats_dict(self.last_stats()); })
+        .def(
+            "render_batch",
+            [](const PathTracer &self, const std::vector<Scene> &scenes, const std::vector<int> &num_samples,
+               const std::vector<uint64_t> &seeds, int max_depth, int rr_depth, bool shadow_bounded, bool hdr, bool fp64,
+               int queue_capacity, int block_size) {
+                RenderOptions o;
+                o.max_depth = max_depth;
+                o.rr_depth = rr_depth;
+                o.shadow_bounded = shadow_bounded;
+                o.hdr = hdr;
+                o.fp64 = fp64;
+                o.queue_capacity = queue_capacity;
+                o.block_size = block_size;
+                py::gil_scoped_release nogil;
+                return self.render_batch(scenes, num_samples, seeds, o);
+            },
+            py::arg("scenes"), py::arg("num_samples"), py::arg("seeds"), py::arg("max_depth") = 5,
+            py::arg("rr_depth") = -1, py::arg("shadow_bounded") = false, py::arg("hdr") = false, py::arg("fp64") = false,
+            py::arg("queue_capacity") = 0, py::arg("block_size") = 0)
+        .def("last_batch_stats", [](const PathTracer &self) {
+            const tpt_batch_stats &s = self.last_batch_stats();
+            py::dict d;
+            d["render_ms"] = s.render_ms;
+            d["tune_ms"] = s.tune_ms;
+            d["paths"] = s.paths;
+            d["queue_capacity"] = s.queue_capacity;
+            d["block_size"] = s.block_size;
+            d["waves"] = s.waves;
+            d["scene_ms"] = self.last_batch_scene_ms();
+            return d;
+        });
 
     // image (pybind.cpp:76-100)
     py::class_<Image, std::shared_ptr<Image>>(m, "Image")
@@ -213,7 +244,8 @@ PYBIND11_MODULE(_C, m) {
         .def("to", [](const Scene &self, const std::string &d) { return self.to(Device(d)); })
         .def_property_readonly("device", [](const Scene &s) { return s.device().str(); })
         .def("_desc_ptr", [](const Scene &s) { return (uintptr_t)s.desc(); })
-        .def("_hip_ptr", [](const Scene &s) { return (uintptr_t)s.hip_handle(); });
+        .def("_hip_ptr", [](const Scene &s) { return (uintptr_t)s.hip_handle(); })
+        .def("camera", [](const Scene &s) { return s.camera(); });
 
     // objects (pybind.cpp:115-135)
     py::class_<Object, std::shared_ptr<Object>>(m, "Object3d");

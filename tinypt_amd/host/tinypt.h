@@ -400,10 +400,20 @@ class PathTracer {
     Image render(const Scene &scene, int num_samples) const { return render(scene, num_samples, RenderOptions{}); }
     Image render(const Scene &scene, int num_samples, const RenderOptions &opt) const;
     const tpt_render_stats &last_stats() const { return _stats; }
+    // Many independent renders through one shared pipeline (no counterpart in the
+    // reference): scene i with num_samples[i] and seeds[i]; `opt` gives the shared
+    // depth / RR / output settings and the queue capacity and workgroup size (0 =
+    // autotuned).  Each image equals render(scenes[i], num_samples[i]) with that seed.
+    std::vector<Image> render_batch(const std::vector<Scene> &scenes, const std::vector<int> &num_samples,
+                                    const std::vector<uint64_t> &seeds, const RenderOptions &opt) const;
+    const tpt_batch_stats &last_batch_stats() const { return _batch_stats; }
+    const std::vector<double> &last_batch_scene_ms() const { return _batch_scene_ms; }
 
   private:
     Device _device;
     mutable tpt_render_stats _stats{};
+    mutable tpt_batch_stats _batch_stats{};
+    mutable std::vector<double> _batch_scene_ms;
 };
 
 // Sobel gradient used by BumpTexture (exposed for tests)

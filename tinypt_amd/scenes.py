@@ -141,14 +141,16 @@ def random_rotation(rng):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def room_materials(rng, n):
+def room_materials(rng, n, lambert_texture=None):
     """Per-instance materials: 60% Lambertian U(0.1,0.9), 15% metal, 10% dielectric IOR U(1.3,1.7),
-    15% GGX roughness U(0.05,0.6) (alpha = roughness, shininess = 2/alpha^2 - 2)."""
+    15% GGX roughness U(0.05,0.6) (alpha = roughness, shininess = 2/alpha^2 - 2).
+    `lambert_texture` replaces the Lambertian albedo (the draws stay the same)."""
     mats = []
     for _ in range(n):
         r = rng.random()
         if r < 0.60:
-            mats.append(material(_C.Lambertian(rgb(rng.uniform(0.1, 0.9, 3)))))
+            a = rgb(rng.uniform(0.1, 0.9, 3))
+            mats.append(material(_C.Lambertian(a if lambert_texture is None else lambert_texture)))
         elif r < 0.75:
             mats.append(material(_C.Metal(rgb(rng.uniform(0.5, 0.95, 3)))))
         elif r < 0.85:
@@ -160,8 +162,22 @@ def room_materials(rng, n):
     return mats
 
 
-def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150):
-    """World-space (vertices, normals-or-None, faces9, face_material, materials-spec) for the room."""
+def _local_uv(lv, is_ico):
+    """Texture coordinates of an instance's local vertices: spherical for the
+    icospheres, per-face planar for the subdivided boxes."""
+    if is_ico:
+        return np.stack([0.5 + np.arctan2(lv[:, 1], lv[:, 0]) / (2 * np.pi),
+                         0.5 + np.arcsin(np.clip(lv[:, 2], -1.0, 1.0)) / np.pi], 1)
+    ax = np.argmax(np.abs(lv), axis=1)
+    a1, a2 = (ax + 1) % 3, (ax + 2) % 3
+    idx = np.arange(len(lv))
+    return np.stack([(lv[idx, a1] + 1) * 0.5, (lv[idx, a2] + 1) * 0.5], 1)
+
+
+def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150, uv=False):
+    """World-space (vertices, normals-or-None, faces9, face_material, materials-spec) for the room;
+    with `uv`, also per-vertex texture coordinates of the instances (the faces then
+    reference them; the room box has none)."""
     rng = np.random.default_rng(seed)
     X, Y, Z = 40.0, 40.0, 20.0
     # room box (inward facing), no vertex normals
@@ -174,6 +190,7 @@ def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150
     rng.shuffle(kinds)
     V = [rv]
     N = [np.zeros_like(rv)]
+    UV = [np.zeros((len(rv), 2))]
     F9 = []
     FM = []
     # room faces: material 0, no normals
@@ -192,13 +209,17 @@ def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150
             lv, lf, ln = box_v, box_f, None
         wv = (lv * s) @ R.T + pos
         V.append(wv)
+        if uv:
+            UV.append(_local_uv(lv, kinds[i] == 0))
         if ln is not None:
             N.append(ln @ R.T)
+        t = nv if uv else None
         for f in lf:
+            ta, tb, tc = (f[0] + t, f[1] + t, f[2] + t) if uv else (-1, -1, -1)
             if ln is not None:
-                F9.append([f[0] + nv, f[0] + nn, -1, f[1] + nv, f[1] + nn, -1, f[2] + nv, f[2] + nn, -1])
+                F9.append([f[0] + nv, f[0] + nn, ta, f[1] + nv, f[1] + nn, tb, f[2] + nv, f[2] + nn, tc])
             else:
-                F9.append([f[0] + nv, -1, -1, f[1] + nv, -1, -1, f[2] + nv, -1, -1])
+                F9.append([f[0] + nv, -1, ta, f[1] + nv, -1, tb, f[2] + nv, -1, tc])
             FM.append(1 + i)
         nv += len(lv)
         if ln is not None:
@@ -207,13 +228,101 @@ def triangle_room_arrays(seed=1, instances=200, ico_level=3, box_n=11, n_ico=150
     normals = np.concatenate(N).astype(np.float32)
     faces9 = np.array(F9, dtype=np.int32)
     fm = np.array(FM, dtype=np.int32)
+    if uv:
+        return verts, normals, faces9, fm, rng, np.concatenate(UV).astype(np.float32)
     return verts, normals, faces9, fm, rng
 
 
-def triangle_room(width=1024, height=768, seed=1):
-    verts, normals, faces9, fm, rng = triangle_room_arrays(seed)
-    mats = [material(_C.Lambertian(rgb((0.75, 0.75, 0.75))))] + room_materials(rng, 200)
+# (icosphere level, box subdivision) pairs of similar face counts: 20 * 4^l vs 12 * n^2
+_ROOM_SIZES = ((0, 1), (1, 3), (2, 5), (3, 10), (4, 21))
+
+
+def room_size_for(tris):
+    """(instances, ico_level, box_n, n_ico) for a room of about `tris` triangles:
+    75% icospheres, the instance size whose count lands closest to 200."""
+    cands = []
+    for lvl, bn in _ROOM_SIZES:
+        per = 0.75 * 20 * 4 ** lvl + 0.25 * 12 * bn * bn
+        inst = max(4, int(round((tris - 12) / per)))
+        cands.append((abs(math.log(inst / 200.0)), inst, lvl, bn))
+    _, inst, lvl, bn = min(cands)
+    return inst, lvl, bn, int(round(0.75 * inst))
+
+
+def triangle_room(width=1024, height=768, seed=1, tris=None):
+    """The seeded triangle room; `tris` picks instance count and tessellation for
+    about that many triangles (None: 200 instances, 264,612 triangles)."""
+    if tris is None:
+        inst, lvl, bn, n_ico = 200, 3, 11, 150
+    else:
+        inst, lvl, bn, n_ico = room_size_for(tris)
+    verts, normals, faces9, fm, rng = triangle_room_arrays(seed, inst, lvl, bn, n_ico)
+    mats = [material(_C.Lambertian(rgb((0.75, 0.75, 0.75))))] + room_materials(rng, inst)
     mesh = _C.TriangleMesh.from_arrays(verts, faces9, mats, fm, normals, None)
+    lights = []
+    for cx, cy in ((10, 10), (30, 10), (10, 30), (30, 30)):
+        r = _C.Rectangle()
+        r.location = [float(cx), float(cy), 19.9]
+        r.dimension = [6.0, 6.0]
+        e = float(rng.uniform(5, 20))
+        r.material = material(_C.Lambertian(rgb((0, 0, 0))), emission=(e, e, e))
+        lights.append(r)
+    cam = _C.Camera()
+    cam.location = [20.0, 1.0, 9.0]
+    cam.rotation = lookat_rotation((0.0, 1.0, -0.15))
+    cam.width, cam.height = width, height
+    cam.fov = math.radians(65.0)
+    return _C.Scene(cam, [mesh] + lights, [], None)
+
+
+def value_noise(rng, size, cells):
+    """size x size bilinear value noise over a (cells + 1)^2 random lattice, in [0, 1]."""
+    g = rng.random((cells + 1, cells + 1))
+    x = np.arange(size) * (cells / size)
+    i = x.astype(np.int64)
+    f = x - i
+    a = g[i][:, i] * (1 - f)[None, :] + g[i][:, i + 1] * f[None, :]
+    b = g[i + 1][:, i] * (1 - f)[None, :] + g[i + 1][:, i + 1] * f[None, :]
+    return a * (1 - f)[:, None] + b * f[:, None]
+
+
+def procedural_textures(seed=3, albedo_size=1024, height_size=512):
+    """(RGBA8 albedo: checker x value noise, alpha 0 where the noise is low;
+    FP32 height map: value noise), both seeded."""
+    rng = np.random.default_rng(seed)
+    n = 0.5 * value_noise(rng, albedo_size, 8) + 0.3 * value_noise(rng, albedo_size, 32) + \
+        0.2 * value_noise(rng, albedo_size, 128)
+    y, x = np.mgrid[0:albedo_size, 0:albedo_size]
+    checker = ((x // 64 + y // 64) % 2).astype(np.float64)
+    c0, c1 = np.array([0.85, 0.75, 0.55]), np.array([0.25, 0.35, 0.6])
+    rgb = (c0 * (1 - checker)[..., None] + c1 * checker[..., None]) * (0.5 + n)[..., None]
+    rgba = np.empty((albedo_size, albedo_size, 4), np.uint8)
+    rgba[..., :3] = np.clip(rgb * 255.0, 0, 255).astype(np.uint8)
+    rgba[..., 3] = np.where(n < 0.38, 0, 255).astype(np.uint8)
+    height = (0.6 * value_noise(rng, height_size, 16) + 0.4 * value_noise(rng, height_size, 64)).astype(np.float32)
+    return rgba, height
+
+
+def textured_room(width=1920, height=1080, seed=1, texture_seed=3, cutout=0.2, bump=0.5):
+    """The 264,612-triangle room with per-vertex texture coordinates, a 1024^2
+    RGBA8 checker / value-noise albedo on its Lambertian and glossy instances,
+    alpha cut-out (the texture's alpha channel) on `cutout` of the instances and a
+    512^2 FP32 height map as bump map on `bump` of them."""
+    verts, normals, faces9, fm, rng, uvs = triangle_room_arrays(seed, uv=True)
+    rgba, hmap = procedural_textures(texture_seed)
+    albedo_img = _C.Image.from_linear(rgba)
+    albedo = _C.RGBTexture(albedo_img)
+    alpha = _C.AlphaTexture(albedo_img, 3)
+    bumpt = _C.BumpTexture(_C.Image.from_linear(hmap), 4.0)
+    mats = [material(_C.Lambertian(rgb((0.75, 0.75, 0.75))))]
+    trng = np.random.default_rng(seed * 7919 + texture_seed)
+    for m in room_materials(rng, 200, albedo):
+        if trng.random() < cutout:
+            m.alpha_texture = alpha
+        if trng.random() < bump:
+            m.bump_texture = bumpt
+        mats.append(m)
+    mesh = _C.TriangleMesh.from_arrays(verts, faces9, mats, fm, normals, uvs)
     lights = []
     for cx, cy in ((10, 10), (30, 10), (10, 30), (30, 30)):
         r = _C.Rectangle()
@@ -233,3 +342,23 @@ def triangle_room(width=1024, height=768, seed=1):
 # presets: (spp, max_depth, rr_depth)
 CONFIG0 = dict(width=512, height=384, spp=64, max_depth=10, rr_depth=5)
 CONFIG1 = dict(width=1024, height=768, spp=256, max_depth=8, rr_depth=5)
+CONFIG2 = dict(count=64, tris=(2000, 500000), side=(128, 1024), spp=(16, 256), max_depth=8, rr_depth=5)
+
+
+def batch_specs(count=64, seed=2024, tris=(2000, 500000), side=(128, 1024), spp=(16, 256)):
+    """Seeded specs of a batch of independent triangle rooms: triangle counts
+    log-uniform in `tris`, square resolutions log-uniform in `side` (multiples of
+    8), samples per pixel uniform in `spp`."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        t = int(round(math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1])))))
+        s = int(round(2.0 ** rng.uniform(math.log2(side[0]), math.log2(side[1])) / 8.0)) * 8
+        n = int(rng.integers(spp[0], spp[1] + 1))
+        specs.append(dict(seed=seed * 1000 + i, tris=t, width=s, height=s, spp=n))
+    return specs
+
+
+def batch_scenes(specs):
+    """Host scenes of `batch_specs` entries."""
+    return [triangle_room(sp["width"], sp["height"], sp["seed"], sp["tris"]) for sp in specs]
