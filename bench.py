@@ -16,6 +16,9 @@ Other workloads (one JSON line each, same fields):
                       queue capacity and workgroup size; per-scene completion
                       times and the same scenes rendered one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
+  --workload cache    the textured room at 1920x1080, depth 12: plain tracer vs the
+                      learned radiance cache at equal relMSE against a plain FP64
+                      reference of --ref-spp (16384) samples per pixel
 """
 import argparse
 import json
@@ -89,7 +92,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", choices=["room", "batch", "spheres"], default="room")
+    ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache"], default="room")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="cache: samples per pixel of the reference")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true", help="batch: skip the one-by-one comparison")
     args = ap.parse_args()
@@ -113,6 +117,11 @@ def main():
         return
     if args.workload == "spheres":
         bench_spheres(args)
+        return
+    if args.workload == "cache":
+        if args.width == 1024 and args.height == 768:
+            args.width, args.height = 1920, 1080
+        bench_cache(args)
         return
     t0 = time.perf_counter()
     host = scenes.triangle_room(args.width, args.height)
@@ -265,6 +274,68 @@ def bench_batch(args):
         seq_s = time.perf_counter() - t
         out["sequential_mpaths_per_s"] = round(paths / seq_s / 1e6, 2)
         out["batch_speedup_vs_sequential"] = round(seq_s * 1e3 / ms_per_step, 3)
+    print(json.dumps(out))
+
+
+def bench_cache(args):
+    """The textured room (RGBA8 albedo + alpha cut-out + FP32 bump maps), depth 12,
+    plain tracer vs the learned radiance cache, both against a plain FP64 reference
+    of --ref-spp samples per pixel.  Plain relMSE falls as c / time (unbiased
+    Monte Carlo), so the plain tracer's time to reach each cached render's relMSE
+    is c / relMSE; the ratio to the cached render's time is the equal-error speedup."""
+    import numpy as np
+
+    import tinypt_amd
+    from tinypt_amd import scenes
+
+    W, H = args.width, args.height
+    dev = scenes.textured_room(W, H).to("hip")
+    pt = tinypt_amd.PathTracer("hip")
+    kw = dict(max_depth=12, rr_depth=5)
+    t = time.perf_counter()
+    ref = pt.render(dev, args.ref_spp, seed=777, hdr=True, fp64=True, **kw).raw()
+    print(f"# reference {args.ref_spp} spp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+
+    def rel(x):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    pt.render(dev, 4, seed=1, hdr=True, **kw)  # warm-up
+    plain = []
+    for spp in (16, 32, 64, 128, 256):
+        t = time.perf_counter()
+        img = pt.render(dev, spp, seed=1, hdr=True, **kw).raw()
+        dt = time.perf_counter() - t
+        plain.append((spp, dt, rel(img)))
+        print(f"# plain {spp:4d} spp: {dt * 1e3:8.1f} ms relMSE {plain[-1][2]:.5f}", file=sys.stderr, flush=True)
+    c = sum(e * dt for _, dt, e in plain) / len(plain)  # relMSE * time of the plain tracer
+    cached = []
+    tinypt_amd.render_cached(dev, 2, cache_depth=2, reset=True, seed=3, **kw)  # warm-up
+    for frames in (16, 32, 64, 128, 256):
+        t = time.perf_counter()
+        img, st = tinypt_amd.render_cached(dev, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3,
+                                           reset=True, weight_seed=5, seed=1, **kw)
+        dt = time.perf_counter() - t
+        e = rel(img)
+        cached.append(dict(frames=frames, ms=round(dt * 1e3, 1), relmse=round(e, 5),
+                           plain_ms_same_relmse=round(c / e * 1e3, 1), speedup=round(c / e / dt, 3),
+                           infer_ms=round(st["infer_ms"], 1), train_ms=round(st["train_ms"], 1),
+                           mean_bias=round(float(img.mean() / ref.mean() - 1.0), 4)))
+        print(f"# cached {frames:4d} frames: {dt * 1e3:8.1f} ms relMSE {e:.5f} -> equal-error speedup "
+              f"{cached[-1]['speedup']:.2f} (bias {cached[-1]['mean_bias']:+.4f})", file=sys.stderr, flush=True)
+    best = max(cached, key=lambda r: r["speedup"])
+    paths = W * H * best["frames"]
+    out = base_line(paths / (best["ms"] / 1e3) / 1e6, best["ms"], args, {
+        "workload": f"textured_room_264612tri_{W}x{H}_depth12_rr5_rgba8_alpha_bump",
+        "tracer": "radiance cache (5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, cache after 2 bounces)",
+        "paths_per_step": paths,
+        "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation",
+    })
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    out["equal_relmse_speedup_vs_plain"] = best["speedup"]
+    out["plain_relmse_times_s"] = round(c, 5)
+    out["plain"] = [dict(spp=s, ms=round(dt * 1e3, 1), relmse=round(e, 5)) for s, dt, e in plain]
+    out["cached"] = cached
     print(json.dumps(out))
 
 

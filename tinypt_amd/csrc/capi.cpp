@@ -892,10 +892,12 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         const int grid = grid_for((size_t)pixels, block, h->cus);
         const uint64_t base_mixed = splitmix64(p->seed);
         const int fp64 = p->accum_fp64 ? 1 : 0;
-        hipEvent_t e[4];
-        for (auto &x : e) HIP_OK(hipEventCreate(&x));
+        // four events per frame (inference and training brackets), read after the last frame
+        std::vector<hipEvent_t> ev(use_cache ? 4 * (size_t)cp->frames : 0);
+        for (auto &x : ev) HIP_OK(hipEventCreate(&x));
         double infer_ms = 0, train_ms = 0;
         for (int f = 0; f < cp->frames; f++) {
+            hipEvent_t *e = use_cache ? &ev[4 * (size_t)f] : nullptr;
             C.frame = (uint32_t)(c.step + f);
             launch_cache_reset(C, st);
             launch_generate(S, R, P, f, pixels, base_mixed, block, st);
@@ -924,12 +926,6 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
                     nrc_launch_adam(c.W, cp->learning_rate, ++c.step, st);
                 }
                 HIP_OK(hipEventRecord(e[3], st));
-                HIP_OK(hipEventSynchronize(e[3]));
-                float a = 0, b2 = 0;
-                HIP_OK(hipEventElapsedTime(&a, e[0], e[1]));
-                HIP_OK(hipEventElapsedTime(&b2, e[2], e[3]));
-                infer_ms += a;
-                train_ms += b2;
             }
             launch_frame_accumulate(P, pixels, fp64, f == 0, st);
         }
@@ -941,7 +937,14 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         HIP_OK(hipMemcpyAsync(cnt, c.counters, 8, hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        for (auto &x : e) (void)hipEventDestroy(x);
+        for (size_t f = 0; 4 * f < ev.size(); f++) {
+            float a = 0, b2 = 0;
+            HIP_OK(hipEventElapsedTime(&a, ev[4 * f], ev[4 * f + 1]));
+            HIP_OK(hipEventElapsedTime(&b2, ev[4 * f + 2], ev[4 * f + 3]));
+            infer_ms += a;
+            train_ms += b2;
+        }
+        for (auto &x : ev) (void)hipEventDestroy(x);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
