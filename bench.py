@@ -16,6 +16,9 @@ Other workloads (one JSON line each, same fields):
                       queue capacity and workgroup size; per-scene completion
                       times and the same scenes rendered one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
+  --workload city     the ~6M-triangle city (HDR sky CDF, sun, 1000 point lights),
+                      1920x1080 at 1024 spp: plain tracer (judge), batched pipeline
+                      and radiance cache
   --workload cache    the textured room at 1920x1080, depth 12: plain tracer vs the
                       learned radiance cache at equal relMSE against a plain FP64
                       reference of --ref-spp (16384) samples per pixel
@@ -92,7 +95,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
-    ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache"], default="room")
+    ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache", "city"], default="room")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache: samples per pixel of the reference")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true", help="batch: skip the one-by-one comparison")
@@ -117,6 +120,9 @@ def main():
         return
     if args.workload == "spheres":
         bench_spheres(args)
+        return
+    if args.workload == "city":
+        bench_city(args)
         return
     if args.workload == "cache":
         if args.width == 1024 and args.height == 768:
@@ -336,6 +342,63 @@ def bench_cache(args):
     out["plain_relmse_times_s"] = round(c, 5)
     out["plain"] = [dict(spp=s, ms=round(dt * 1e3, 1), relmse=round(e, 5)) for s, dt, e in plain]
     out["cached"] = cached
+    print(json.dumps(out))
+
+
+def bench_city(args):
+    """The ~6M-triangle city, 1920x1080 at 1024 spp, depth 8, RR 5: the plain tracer
+    (the judge: FP64 accumulation), the batched pipeline (one captured pipeline
+    wave replayed) and the learned radiance cache, each timed; relMSE against
+    the plain 1024-spp render at equal sample counts."""
+    import numpy as np
+
+    import tinypt_amd
+    from tinypt_amd import scenes
+
+    c = scenes.CONFIG4
+    t = time.perf_counter()
+    host = scenes.city(c["width"], c["height"])
+    dev = host.to("hip")
+    info = tinypt_amd.scene_info(dev)
+    print(f"# city ready in {time.perf_counter() - t:.1f} s: {info}", file=sys.stderr, flush=True)
+    pt = tinypt_amd.PathTracer("hip")
+    kw = dict(max_depth=c["max_depth"], rr_depth=c["rr_depth"])
+    pt.render(dev, 4, seed=1, hdr=True, **kw)  # warm-up
+    t = time.perf_counter()
+    ref = pt.render(dev, c["spp"], seed=1, hdr=True, fp64=True, **kw).raw()
+    t_ref = time.perf_counter() - t
+    st = pt.last_stats()
+    print(f"# plain {c['spp']} spp (judge): {t_ref:.1f} s", file=sys.stderr, flush=True)
+
+    def rel(x):
+        return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+    rows = {}
+    spp = 64
+    t = time.perf_counter()
+    img = pt.render(dev, spp, seed=2, hdr=True, **kw).raw()
+    rows["plain_64spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(img), 5))
+    t = time.perf_counter()
+    imgs = pt.render_batch([dev], [spp], [2], hdr=True, **kw)
+    bst = pt.last_batch_stats()
+    rows["batch_64spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(imgs[0].raw()), 5),
+                               equals_plain=bool(np.array_equal(imgs[0].raw(), img)),
+                               queue_capacity=bst["queue_capacity"], block_size=bst["block_size"])
+    t = time.perf_counter()
+    cimg, cst = tinypt_amd.render_cached(dev, spp, cache_depth=2, train_records=1 << 16, reset=True, seed=2, **kw)
+    rows["cached_64frames"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(cimg), 5),
+                                   mean_bias=round(float(cimg.mean() / ref.mean() - 1.0), 4))
+    for k, v in rows.items():
+        print(f"# {k}: {v}", file=sys.stderr, flush=True)
+    out = base_line(st["paths"] / t_ref / 1e6, t_ref * 1e3, args, {
+        "workload": f"city_{info['triangles']}tri_{c['width']}x{c['height']}_{c['spp']}spp_depth8_rr5_"
+                    "hdr_sky_cdf_sun_1000_point_lights",
+        "tracer": "plain HIP wavefront (judge, FP64 accumulation)",
+        "paths_per_step": st["paths"],
+    })
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    out["at_64spp_vs_judge"] = rows
     print(json.dumps(out))
 
 

@@ -339,6 +339,119 @@ def textured_room(width=1920, height=1080, seed=1, texture_seed=3, cutout=0.2, b
     return _C.Scene(cam, [mesh] + lights, [], None)
 
 
+def procedural_sky(width=2048, height=1024, seed=11):
+    """RGBA32F equirectangular HDR sky (row 0 = bottom = nadir, as the texture
+    convention): a horizon-to-zenith gradient with value-noise clouds and a bright
+    sun disk; the last channel is 1."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(height) + 0.5) / height            # 0 = nadir, 1 = zenith
+    u = (np.arange(width) + 0.5) / width
+    elev = (v - 0.5) * np.pi                          # -pi/2 .. pi/2
+    up = np.clip(np.sin(elev), 0.0, 1.0)[:, None]
+    horizon = np.array([0.9, 0.85, 0.8]) * 1.5
+    zenith = np.array([0.25, 0.45, 0.95]) * 1.2
+    ground = np.array([0.15, 0.13, 0.12])
+    sky = horizon * (1 - up[..., None]) + zenith * up[..., None]
+    sky = np.where((elev < 0)[:, None, None], ground, sky) * np.ones((height, width, 1))
+    clouds = value_noise(rng, max(width, height), 24)[:height, :width]
+    sky = sky * (1.0 + 0.6 * np.clip(clouds - 0.5, 0, 1)[..., None] * (up[..., None] > 0))
+    # sun disk (elevation 35 deg, azimuth at u = 0.3)
+    su, se = 0.3, math.radians(35.0)
+    d = np.sqrt(((u[None, :] - su) * 2 * np.pi * np.cos(elev[:, None])) ** 2 + (elev[:, None] - se) ** 2)
+    sky = sky + (d < 0.02)[..., None] * np.array([2000.0, 1800.0, 1500.0])
+    out = np.ones((height, width, 4), np.float32)
+    out[..., :3] = sky.astype(np.float32)
+    return out
+
+
+def city_arrays(seed=21, target_tris=6_000_000, grid=512, extent=2000.0):
+    """World-space (vertices, faces (F, 3), face material, material count) of a
+    seeded city: a grid x grid displaced heightfield (2 (grid - 1)^2 triangles)
+    plus extruded box buildings (10 triangles each, no bottom) with log-normal
+    heights of mean 30, filling the rest of `target_tris`."""
+    rng = np.random.default_rng(seed)
+    # heightfield
+    h = 25.0 * value_noise(rng, grid, 8) + 5.0 * value_noise(rng, grid, 48)
+    xs = np.linspace(0.0, extent, grid)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    tv = np.stack([X.ravel(), Y.ravel(), h.ravel()], 1)
+    i, j = np.meshgrid(np.arange(grid - 1), np.arange(grid - 1), indexing="xy")
+    a = (j * grid + i).ravel()
+    b, c, d = a + 1, a + grid + 1, a + grid
+    tf = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)])
+    # buildings on a jittered grid of lots
+    nb = max(0, (target_tris - len(tf)) // 10)
+    side = int(math.ceil(math.sqrt(nb)))
+    lot = extent / side
+    k = np.arange(nb)
+    cx = (k % side + 0.5) * lot + rng.uniform(-0.15, 0.15, nb) * lot
+    cy = (k // side + 0.5) * lot + rng.uniform(-0.15, 0.15, nb) * lot
+    hw = rng.uniform(0.25, 0.42, (nb, 2)) * lot
+    sigma = 0.6
+    height = rng.lognormal(math.log(30.0) - sigma * sigma / 2, sigma, nb)
+    gi = np.clip((cx / extent * (grid - 1)).astype(np.int64), 0, grid - 1)
+    gj = np.clip((cy / extent * (grid - 1)).astype(np.int64), 0, grid - 1)
+    base = h[gj, gi] - 1.0
+    corners = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], np.float64)
+    bx = cx[:, None] + corners[None, :, 0] * hw[:, 0:1]
+    by = cy[:, None] + corners[None, :, 1] * hw[:, 1:2]
+    bot = np.stack([bx, by, np.repeat(base[:, None], 4, 1)], 2)
+    top = np.stack([bx, by, np.repeat((base + height)[:, None], 4, 1)], 2)
+    bv = np.concatenate([bot, top], 1).reshape(-1, 3)  # 8 vertices per building
+    box = np.array([[4, 5, 6], [4, 6, 7],                      # roof
+                    [0, 1, 5], [0, 5, 4], [1, 2, 6], [1, 6, 5],  # walls
+                    [2, 3, 7], [2, 7, 6], [3, 0, 4], [3, 4, 7]], np.int64)
+    bf = (box[None, :, :] + (len(tv) + 8 * k)[:, None, None]).reshape(-1, 3)
+    verts = np.concatenate([tv, bv]).astype(np.float32)
+    faces = np.concatenate([tf, bf]).astype(np.int32)
+    n_mat = 64
+    fm = np.concatenate([np.zeros(len(tf), np.int32),
+                         np.repeat(rng.integers(1, n_mat, nb).astype(np.int32), 10)])
+    return verts, faces, fm, n_mat, rng
+
+
+def city(width=1920, height=1080, seed=21, target_tris=6_000_000, point_lights=1000, importance=True):
+    """The configs' city: ~6M triangles, a 2048x1024 HDR sky sampled through its
+    CDF (`importance`), one sun (distant light) and `point_lights` point lights;
+    the terrain is diffuse, the buildings use a palette of GGX and dielectric
+    materials."""
+    verts, faces, fm, n_mat, rng = city_arrays(seed, target_tris)
+    mats = [material(_C.Lambertian(rgb((0.35, 0.33, 0.3))))]
+    for _ in range(n_mat - 1):
+        if rng.random() < 0.7:
+            rough = float(rng.uniform(0.1, 0.6))
+            shin = 2.0 / (rough * rough) - 2.0
+            mats.append(material(_C.Glossy(shin, rgb(rng.uniform(0.3, 0.9, 3)), _C.Glossy.MICRO_TROWBRIDGE_REITZ)))
+        else:
+            mats.append(material(_C.Dielectric(rgb((0.95, 0.97, 0.99)), float(rng.uniform(1.4, 1.6)))))
+    mesh = _C.TriangleMesh.from_arrays(verts, faces, mats, fm)
+    lights = []
+    sun = _C.DistantLight()
+    sun.rotation = _C.rotation_from_direction([math.cos(math.radians(35)) * math.cos(0.3 * 2 * math.pi),
+                                               math.cos(math.radians(35)) * math.sin(0.3 * 2 * math.pi),
+                                               math.sin(math.radians(35))])
+    sun.power = 3.0
+    sun.color = [1.0, 0.95, 0.85]
+    lights.append(sun)
+    ext = float(verts[:, 0].max())
+    for _ in range(point_lights):
+        pl = _C.PointLight()
+        pl.location = [float(rng.uniform(0, ext)), float(rng.uniform(0, ext)), float(rng.uniform(8.0, 20.0))]
+        pl.color = [1.0, 0.8, 0.55]
+        pl.power = float(rng.uniform(0.5, 2.0))
+        lights.append(pl)
+    bg = _C.EnvTexture(_C.RGBTexture(_C.Image.from_linear(procedural_sky())), importance)
+    cam = _C.Camera()
+    cam.location = [ext * 0.1, ext * 0.1, 180.0]
+    cam.rotation = lookat_rotation((1.0, 1.0, -0.35))
+    cam.width, cam.height = width, height
+    cam.fov = math.radians(60.0)
+    return _C.Scene(cam, [mesh], lights, bg)
+
+
+CONFIG4 = dict(width=1920, height=1080, spp=1024, max_depth=8, rr_depth=5)
+
+
 # presets: (spp, max_depth, rr_depth)
 CONFIG0 = dict(width=512, height=384, spp=64, max_depth=10, rr_depth=5)
 CONFIG1 = dict(width=1024, height=768, spp=256, max_depth=8, rr_depth=5)
