@@ -193,9 +193,11 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.persistent = 2;
     if (pers && std::string(pers) == "0") S.persistent = 0;
     if (pers && std::string(pers) == "1") S.persistent = 1;
-    // traversal stack: private scratch by default; TPT_LDS_STACK=1 stages it in LDS
+    // traversal stack: the first kLdsStack entries in LDS by default (the private
+    // scratch stack spills to memory: ~9x the extend kernel's HBM writes);
+    // TPT_LDS_STACK=0 selects the private stack
     const char *lds = std::getenv("TPT_LDS_STACK");
-    S.lds_stack = (lds && std::string(lds) == "1") ? 1 : 0;
+    S.lds_stack = (lds && std::string(lds) == "0") ? 0 : 1;
     // persistent traversal workgroups and the LDS node cache of the quantized layout
     // (the top of the tree, in BFS order at the start of the node array)
     S.trav_block = env_int("TPT_TRAV_BLOCK", kTravBlock, 64, 1024) / 64 * 64;
