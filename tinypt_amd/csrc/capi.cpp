@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -32,7 +33,7 @@ void launch_finalize(const DRender &, const DPaths &, float *, hipStream_t);
 void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, float *, uint32_t *, bool, int, int,
                           hipStream_t);
 void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
-void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, int, uint64_t, hipStream_t);
+void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, int, uint64_t, hipStream_t);
 void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
 void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
 void launch_cache_reset(const DCache &, hipStream_t);
@@ -266,7 +267,7 @@ void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
     size_t o_rng = take(slots * 8), o_key = take(slots * 8), o_dep = take(slots * 4);
     size_t o_qa = take(slots * 4), o_qb = take(slots * 4), o_ss = take(slots * 4);
     size_t o_so = take(slots * 16), o_sd = take(slots * 16), o_sv = take(slots * 16), o_hit = take(slots * 16);
-    size_t o_cnt = take(64);
+    size_t o_cnt = take(kCntWords * 4);
     size_t o_sf = take((size_t)pixels * 12), o_sdd = take((size_t)pixels * 24), o_pf = take((size_t)pixels * 12),
            o_pd = take((size_t)pixels * 24), o_out = take((size_t)pixels * 12);
     size_t o_sid = with_sid ? take(slots * 4) : 0;
@@ -297,7 +298,7 @@ void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
     P.pix_f = (float *)(b + o_pf);
     P.pix_d = (double *)(b + o_pd);
     pb.out = (float *)(b + o_out);
-    HIP_OK(hipMemset(P.counters, 0, 64));
+    HIP_OK(hipMemset(P.counters, 0, kCntWords * 4));
 }
 
 void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
@@ -425,7 +426,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         uint64_t base_mixed = splitmix64(p->seed);
         bool timing = stats != nullptr;
         bool count = p->count_traffic != 0;
-        if (count) HIP_OK(hipMemsetAsync(P.counters + 4, 0, 48, st));
+        if (count) HIP_OK(hipMemsetAsync(P.counters + kCntTraffic, 0, 32, st));
         std::vector<hipEvent_t> evs;
         auto ev = [&]() {
             hipEvent_t e;
@@ -486,7 +487,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             stats->extend_launches = ext_ev.size();
             if (count) {
                 uint64_t c[4];
-                HIP_OK(hipMemcpy(c, P.counters + 4, 32, hipMemcpyDeviceToHost));
+                HIP_OK(hipMemcpy(c, P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
                 stats->nodes_visited = c[0];
                 stats->tris_tested = c[1];
                 stats->rays_closest = c[2];
@@ -513,6 +514,7 @@ struct BatchRun {
     uint64_t total_paths = 0, total_pixels = 0;
     std::vector<uint64_t> path_off, pix_off;
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
+    bool lds_stack = true; // first traversal-stack entries of each lane in LDS
     hipStream_t st = nullptr;
     ~BatchRun() {
         if (st) (void)hipStreamSynchronize(st); // nothing queued may still use the buffers
@@ -525,7 +527,7 @@ struct BatchRun {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, num_cu, total_pixels, st);
+        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, num_cu, total_pixels, st);
         HIP_OK(hipStreamEndCapture(st, &g));
         hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -616,6 +618,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         run.max_depth = p->max_depth;
         run.layout = scenes[0]->S.layout;
         run.trav_block = scenes[0]->S.trav_block;
+        run.lds_stack = scenes[0]->S.lds_stack != 0;
         run.num_cu = scenes[0]->cus;
         run.fp64 = p->accum_fp64 ? 1 : 0;
         std::vector<DScene> table(n);
@@ -623,10 +626,18 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         std::vector<uint64_t> seeds(n);
         run.path_off.assign(n + 1, 0);
         run.pix_off.assign(n + 1, 0);
+        // the device table stores every array as a byte offset from the lowest arena
+        uintptr_t base = (uintptr_t)scenes[0]->arena;
+        for (int i = 1; i < n; i++) base = std::min(base, (uintptr_t)scenes[i]->arena);
+        auto to_offset = [&](auto &ptr) { ptr = (std::remove_reference_t<decltype(ptr)>)((uintptr_t)ptr - base); };
         for (int i = 0; i < n; i++) {
             table[i] = scenes[i]->S;
+            DScene &T = table[i];
+            to_offset(T.nodes), to_offset(T.tris_bvh), to_offset(T.tris), to_offset(T.tri_info);
+            to_offset(T.normals), to_offset(T.texcoords), to_offset(T.gid_local), to_offset(T.analytic);
+            to_offset(T.materials), to_offset(T.images), to_offset(T.texels), to_offset(T.delta);
+            to_offset(T.lights), to_offset(T.env_marg), to_offset(T.env_cond);
             table[i].lds_nodes = 0; // no LDS node cache: wavefronts of one workgroup trace different scenes
-            table[i].lds_stack = 0;
             int px = table[i].cam.width * table[i].cam.height;
             rend[i] = make_render(p, items[i].spp, items[i].seed, px);
             seeds[i] = splitmix64(items[i].seed);
@@ -643,7 +654,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             return o;
         };
         size_t o_sc = take(sizeof(DScene) * n), o_r = take(sizeof(DRender) * n), o_po = take(8 * (n + 1)),
-               o_xo = take(8 * (n + 1)), o_sd = take(8 * n), o_sg = take(4 * (size_t)kSegFields * n), o_st = take(16);
+               o_xo = take(8 * (n + 1)), o_sd = take(8 * n), o_sg = take(4 * (size_t)kSegFields * seg_stride_for(n)), o_st = take(16);
         HIP_OK(hipMalloc(&run.meta, off));
         uint8_t *m = (uint8_t *)run.meta;
         HIP_OK(hipMemcpy(m + o_sc, table.data(), sizeof(DScene) * n, hipMemcpyHostToDevice));
@@ -651,9 +662,10 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         HIP_OK(hipMemcpy(m + o_po, run.path_off.data(), 8 * (n + 1), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(m + o_xo, run.pix_off.data(), 8 * (n + 1), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(m + o_sd, seeds.data(), 8 * n, hipMemcpyHostToDevice));
-        HIP_OK(hipMemsetAsync(m + o_sg, 0, 4 * (size_t)kSegFields * n, run.st));
+        HIP_OK(hipMemsetAsync(m + o_sg, 0, 4 * (size_t)kSegFields * seg_stride_for(n), run.st));
         DBatch &B = run.B;
         B.scenes = (const DScene *)(m + o_sc);
+        B.base = (const uint8_t *)base;
         B.renders = (const DRender *)(m + o_r);
         B.path_off = (const uint64_t *)(m + o_po);
         B.pix_off = (const uint64_t *)(m + o_xo);
@@ -661,6 +673,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.seg = (int32_t *)(m + o_sg);
         B.state = (uint64_t *)(m + o_st);
         B.n = n;
+        B.seg_stride = seg_stride_for(n);
         // candidate capacities: 2^18 .. 2^22 path slots, never (much) more than the batch needs
         auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
         std::vector<int> caps;

@@ -183,13 +183,27 @@ struct DPaths {
     float4 *shadow_dir;  // xyz
     float4 *shadow_val;  // contribution
     float4 *hit;         // t, gid(bits), u, v — indexed by queue position
-    uint32_t *counters;  // [0] queue_a count, [1] queue_b count, [2] shadow count, [3..] traffic counters
+    uint32_t *counters;  // kCntWords words, see kCntQueue
     // per pixel accumulation
     float *sub_f;   // 3 per pixel: running sub-pixel sum (fp32)
     double *sub_d;  // 3 per pixel (fp64 mode)
     float *pix_f;   // 3 per pixel: sum of clipped sub-pixel means (or hdr sum)
     double *pix_d;  // 3 per pixel (fp64 hdr)
     int32_t *sid;   // batch mode: scene of each slot
+};
+
+// Words of DPaths.counters.  Every counter that kernels bump with atomics sits
+// on its own 256-B line: lanes read the queue counts while other waves append
+// to the next queue, and a shared line would queue those reads behind the
+// atomics at one L2 channel.
+enum : int {
+    kCntLine = 64,             // u32 words per 256-B line
+    kCntQueue = 0,             // ray-queue counts: queue q (0 queue_a, 1 queue_b) at q * kCntLine
+    kCntShadow = 2 * kCntLine, // shadow-queue count
+    kCntFetch = 3 * kCntLine,  // refill counter of the persistent extend kernel
+    kCntShFetch = 4 * kCntLine, // refill counter of the persistent shadow kernel
+    kCntTraffic = 5 * kCntLine, // 4 x u64: nodes visited, triangles tested, closest rays, shadow rays
+    kCntWords = 6 * kCntLine
 };
 
 // Learned radiance cache hooks of the shade kernel.  At the path vertex of depth
@@ -227,15 +241,24 @@ enum {
     kSegFields
 };
 struct DBatch {
-    const DScene *scenes;       // n
+    const DScene *scenes;       // n; array pointers stored as byte offsets from `base`
+    const uint8_t *base;        // lowest scene arena of the batch
     const DRender *renders;     // n: per-scene sampler (spp_eff, block, nsub, pixels) + shared depth / RR / output
     const uint64_t *path_off;   // n + 1
     const uint64_t *pix_off;    // n + 1: first pixel of each scene in the concatenated accumulators / output
     const uint64_t *seed_mixed; // n: splitmix64(seed) of each scene
-    int32_t *seg;               // kSegFields * n, field-major: seg[f * n + s]
+    int32_t *seg;               // kSegFields * seg_stride, field-major: seg[f * seg_stride + s]
     uint64_t *state;            // [0] g0 of the current pipeline wave, [1] total paths
     int32_t n;
     int32_t cap;
+    int32_t seg_stride;         // n rounded up to kSegAlign: every field starts on its own cache lines
+    int32_t pad_;
 };
+// Segment fields are kept on separate 256-B lines: the counters that a kernel
+// bumps with atomics would otherwise share a line with the bases and counts
+// that every lane of the same kernel reads, and those reads would queue behind
+// the atomics at one L2 channel.
+static constexpr int kSegAlign = 64;
+inline int seg_stride_for(int n) { return (n + kSegAlign - 1) / kSegAlign * kSegAlign; }
 
 } // namespace tpt

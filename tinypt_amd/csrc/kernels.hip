@@ -559,30 +559,30 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
 
 template <bool COUNT, int NL, bool POST, bool LDS>
 __global__ TPT_TRAVERSAL_ATTR void k_extend_pt(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
-    uint32_t n = P.counters[qsel];
+    uint32_t n = P.counters[qsel * kCntLine];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.counters[nsel_next] = 0u; // next ray queue
-        P.counters[2] = 0u;         // shadow queue
-        P.counters[13] = 0u;        // shadow fetch counter
-        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 8), (unsigned long long)n);
+        P.counters[nsel_next * kCntLine] = 0u; // next ray queue
+        P.counters[kCntShadow] = 0u;
+        P.counters[kCntShFetch] = 0u;
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 4), (unsigned long long)n);
     }
     uint32_t nodes = 0, tris = 0;
     SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
-    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, n, P.counters + 12, nodes, tris);
+    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, n, P.counters + kCntFetch, nodes, tris);
     if (COUNT) {
-        wave_count(nodes, P.counters + 4);
-        wave_count(tris, P.counters + 6);
+        wave_count(nodes, P.counters + kCntTraffic);
+        wave_count(tris, P.counters + kCntTraffic + 2);
     }
 }
 
 template <bool COUNT, int NL, bool POST, bool LDS>
 __global__ TPT_TRAVERSAL_ATTR void k_shadow_pt(DScene S, DPaths P) {
-    uint32_t n = P.counters[2];
+    uint32_t n = P.counters[kCntShadow];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 10), (unsigned long long)n);
+        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
     uint32_t nodes = 0, tris = 0;
     SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
-    persistent_trace<true, false, NL, POST, LDS>(S, src, n, P.counters + 13, nodes, tris);
+    persistent_trace<true, false, NL, POST, LDS>(S, src, n, P.counters + kCntShFetch, nodes, tris);
 }
 
 template <bool ANY, bool COUNT, int NL, bool POST, bool LDS>
@@ -636,8 +636,8 @@ __device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, cons
 __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_paths, uint64_t base_mixed) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) {
-        P.counters[0] = (uint32_t)n_paths;
-        P.counters[12] = 0u; // extend fetch counter
+        P.counters[kCntQueue] = (uint32_t)n_paths;
+        P.counters[kCntFetch] = 0u;
     }
     if (j >= n_paths) return;
     gen_path(S, R, P, j, j % R.pixels, wave_base + j / R.pixels, base_mixed);
@@ -646,12 +646,12 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
 // ---------------------------------------------------------------- extend
 template <bool COUNT, int NL>
 __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
-    uint32_t n = P.counters[qsel];
+    uint32_t n = P.counters[qsel * kCntLine];
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
-        P.counters[nsel_next] = 0u; // next ray queue
-        P.counters[2] = 0u;         // shadow queue
-        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 8), (unsigned long long)n);
+        P.counters[nsel_next * kCntLine] = 0u; // next ray queue
+        P.counters[kCntShadow] = 0u;
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 4), (unsigned long long)n);
     }
     uint32_t nodes = 0, tris = 0;
     for (uint32_t i = (uint32_t)tid; i < n; i += gridDim.x * blockDim.x) {
@@ -662,8 +662,8 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
         P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
     }
     if (COUNT) {
-        wave_count(nodes, P.counters + 4);
-        wave_count(tris, P.counters + 6);
+        wave_count(nodes, P.counters + kCntTraffic);
+        wave_count(tris, P.counters + kCntTraffic + 2);
     }
 }
 
@@ -871,8 +871,8 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
 template <bool CACHE>
 __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel,
                         DCache C) {
-    uint32_t n = P.counters[qsel];
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[12] = 0u; // fetch counter of the next extend
+    uint32_t n = P.counters[qsel * kCntLine];
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[kCntFetch] = 0u; // refill counter of the next extend
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         bool valid = i < n;
@@ -885,9 +885,9 @@ __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int
             int tp = wave_append(r.cache_kind == 2, C.counters + 1);
             if (r.cache_kind == 2 && tp < C.train_cap) C.t_slot[tp] = slot;
         }
-        int pos = wave_append(r.alive, P.counters + nsel);
+        int pos = wave_append(r.alive, P.counters + nsel * kCntLine);
         if (r.alive) next[pos] = slot;
-        int spos = wave_append(r.emit_shadow, P.counters + 2);
+        int spos = wave_append(r.emit_shadow, P.counters + kCntShadow);
         if (r.emit_shadow) {
             P.shadow_slot[spos] = slot;
             P.shadow_org[spos] = make_float4(r.s_org.x, r.s_org.y, r.s_org.z, r.s_tmax);
@@ -900,9 +900,9 @@ __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int
 // ---------------------------------------------------------------- shadow
 template <bool COUNT, int NL>
 __global__ void k_shadow(DScene S, DPaths P) {
-    uint32_t n = P.counters[2];
+    uint32_t n = P.counters[kCntShadow];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + 10), (unsigned long long)n);
+        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
     uint32_t nodes = 0, tris = 0;
     AlphaCtx ac{0, 0, false};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1075,7 +1075,51 @@ __global__ void k_frame_finalize(DPaths P, int pixels, int fp64, int frames, flo
 // counterparts of generate / extend / shade / shadow / accumulate, built on the
 // same per-path device functions, so every scene's image equals its
 // single-scene render bit for bit.
-#define TPT_SEG(B, f, s) (B).seg[(f) * (B).n + (s)]
+#define TPT_SEG(B, f, s) (B).seg[(f) * (B).seg_stride + (s)]
+
+// Scene s of the batch.  The table stores each array as a byte offset from
+// B.base (a kernel argument): a pointer read from memory would be generic and
+// compile to flat loads, while base + offset stays a global pointer, so every
+// material, texture and node access is a global load as in the single-scene
+// kernels.
+template <class T> __device__ __forceinline__ T *rebase(const uint8_t *base, T *off) {
+    return (T *)(base + (uintptr_t)off);
+}
+// The scene and sampler tables are read through the constant address space:
+// those loads are invariant, so with a wave-uniform index they become scalar
+// loads into SGPRs, like a kernel-argument record.  Through a generic pointer
+// they would be vector loads repeated at every use, adding a dependent memory
+// round trip in front of each scene-array access.
+template <class T> __device__ __forceinline__ T load_invariant(const T *p) {
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "word-aligned records only");
+    typedef const __attribute__((address_space(4))) uint32_t const_u32;
+    const_u32 *src = (const_u32 *)p;
+    T v;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = src[k];
+    return v;
+}
+__device__ __forceinline__ DRender render_at(const DBatch &B, int s) { return load_invariant(B.renders + s); }
+__device__ __forceinline__ DScene scene_at(const DBatch &B, int s) {
+    DScene S = load_invariant(B.scenes + s);
+    S.nodes = rebase(B.base, S.nodes);
+    S.tris_bvh = rebase(B.base, S.tris_bvh);
+    S.tris = rebase(B.base, S.tris);
+    S.tri_info = rebase(B.base, S.tri_info);
+    S.normals = rebase(B.base, S.normals);
+    S.texcoords = rebase(B.base, S.texcoords);
+    S.gid_local = rebase(B.base, S.gid_local);
+    S.analytic = rebase(B.base, S.analytic);
+    S.materials = rebase(B.base, S.materials);
+    S.images = rebase(B.base, S.images);
+    S.texels = rebase(B.base, S.texels);
+    S.delta = rebase(B.base, S.delta);
+    S.lights = rebase(B.base, S.lights);
+    S.env_marg = rebase(B.base, S.env_marg);
+    S.env_cond = rebase(B.base, S.env_cond);
+    return S;
+}
 
 // Largest s in [0, n) with off[s] <= v (off non-decreasing, off[0] <= v).
 template <class T> __device__ __forceinline__ int find_seg(const T *off, int n, T v) {
@@ -1113,7 +1157,8 @@ __global__ void k_generate_batch(DBatch B, DPaths P) {
     int s = find_seg(B.path_off, B.n, g);
     const DRender &R = B.renders[s];
     uint64_t l = g - B.path_off[s];
-    gen_path(B.scenes[s], R, P, (int)j, (int)(l % (uint64_t)R.pixels), (int)(l / (uint64_t)R.pixels), B.seed_mixed[s]);
+    gen_path(scene_at(B, s), R, P, (int)j, (int)(l % (uint64_t)R.pixels), (int)(l / (uint64_t)R.pixels),
+             B.seed_mixed[s]);
     P.sid[j] = s;
 }
 
@@ -1127,7 +1172,7 @@ __device__ __forceinline__ int batch_start_scene(const DBatch &B) {
     return (int)(((int64_t)wave * B.n) / nwaves);
 }
 
-template <int NL>
+template <int NL, bool LDS>
 __global__ TPT_TRAVERSAL_ATTR void k_extend_batch(DBatch B, DPaths P, int qsel) {
     const int fb = qsel ? kSegQbBase : kSegQaBase;
     const int32_t *queue = qsel ? P.queue_b : P.queue_a;
@@ -1140,8 +1185,8 @@ __global__ TPT_TRAVERSAL_ATTR void k_extend_batch(DBatch B, DPaths P, int qsel) 
         if (cnt == 0) continue;
         int base = TPT_SEG(B, fb, s);
         SrcQueue src{queue + base, P.org, P.dir, P.key, P.depth, P.hit + base};
-        persistent_trace<false, false, NL, true, false>(B.scenes[s], src, (uint32_t)cnt,
-                                                        (uint32_t *)&TPT_SEG(B, kSegFetch, s), nodes, tris);
+        persistent_trace<false, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt,
+                                                      (uint32_t *)&TPT_SEG(B, kSegFetch, s), nodes, tris);
     }
 }
 
@@ -1174,11 +1219,14 @@ __global__ void k_batch_seg(DBatch B, int qsel) {
     }
 }
 
-__global__ void k_shade_batch(DBatch B, DPaths P, int qsel) {
+// Workgroups are at most 512 threads (the autotuner's range), which lets the
+// compiler budget registers for that size instead of 1024.
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 512))) void
+k_shade_batch(DBatch B, DPaths P, int qsel) {
     const int cb = qsel ? kSegQbBase : kSegQaBase, nb = qsel ? kSegQaBase : kSegQbBase;
     const int32_t *queue = qsel ? P.queue_b : P.queue_a;
     int32_t *next = qsel ? P.queue_a : P.queue_b;
-    const int32_t *cbase = B.seg + cb * B.n;
+    const int32_t *cbase = B.seg + cb * B.seg_stride;
     for (uint32_t base = blockIdx.x * blockDim.x; base < (uint32_t)B.cap; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
         int s = 0;
@@ -1195,7 +1243,12 @@ __global__ void k_shade_batch(DBatch B, DPaths P, int qsel) {
             int s0 = __builtin_amdgcn_readlane(s, lead);
             bool mine = valid && s == s0;
             ShadeOut r{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
-            if (mine) r = shade_entry(B.scenes[s0], B.renders[s0], P, i, slot);
+            if (mine) {
+                // `mine` implies s == s0, and the compiler may then index the tables
+                // with the per-lane s; readfirstlane keeps the index in an SGPR
+                int su = __builtin_amdgcn_readfirstlane(s0);
+                r = shade_entry(scene_at(B, su), render_at(B, su), P, i, slot);
+            }
             int pos = wave_append(r.alive, (uint32_t *)&TPT_SEG(B, nb + 1, s0));
             if (r.alive) next[TPT_SEG(B, nb, s0) + pos] = slot;
             int spos = wave_append(r.emit_shadow, (uint32_t *)&TPT_SEG(B, kSegShCnt, s0));
@@ -1211,7 +1264,7 @@ __global__ void k_shade_batch(DBatch B, DPaths P, int qsel) {
     }
 }
 
-template <int NL>
+template <int NL, bool LDS>
 __global__ TPT_TRAVERSAL_ATTR void k_shadow_batch(DBatch B, DPaths P) {
     const int start = batch_start_scene(B);
     uint32_t nodes = 0, tris = 0;
@@ -1222,8 +1275,8 @@ __global__ TPT_TRAVERSAL_ATTR void k_shadow_batch(DBatch B, DPaths P) {
         if (cnt == 0) continue;
         int base = TPT_SEG(B, kSegShBase, s);
         SrcShadow src{P.shadow_org + base, P.shadow_dir + base, P.shadow_val + base, P.shadow_slot + base, P.color};
-        persistent_trace<true, false, NL, true, false>(B.scenes[s], src, (uint32_t)cnt,
-                                                       (uint32_t *)&TPT_SEG(B, kSegShFetch, s), nodes, tris);
+        persistent_trace<true, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt,
+                                                     (uint32_t *)&TPT_SEG(B, kSegShFetch, s), nodes, tris);
     }
 }
 
@@ -1413,10 +1466,10 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
 // device memory, so the sequence can be captured once in a hipGraph and replayed
 // ceil(total / cap) times.
 void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layout, int block, int trav_block,
-                       int num_cu, uint64_t total_pixels, hipStream_t st) {
-    auto cu_grid = [&](const void *k, int tb) {
+                       bool lds_stack, int num_cu, uint64_t total_pixels, hipStream_t st) {
+    auto cu_grid = [&](const void *k, int tb, size_t sh) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, tb, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, tb, sh) != hipSuccess || per_cu < 1) per_cu = 1;
         return num_cu * per_cu;
     };
     int gen_grid = (B.cap + block - 1) / block;
@@ -1424,16 +1477,21 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
     by_layout(layout, [&](auto L) {
-        constexpr int NL = decltype(L)::value;
-        int eg = cu_grid((const void *)(k_extend_batch<NL>), trav_block);
-        int sg = cu_grid((const void *)(k_shadow_batch<NL>), trav_block);
-        for (int b = 0; b <= max_depth; b++) {
-            int qsel = b & 1;
-            hipLaunchKernelGGL((k_extend_batch<NL>), dim3(eg), dim3(trav_block), 0, st, B, P, qsel);
-            hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
-            hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
-            hipLaunchKernelGGL((k_shadow_batch<NL>), dim3(sg), dim3(trav_block), 0, st, B, P);
-        }
+        by_bool(lds_stack, [&](auto K) {
+            constexpr int NL = decltype(L)::value;
+            constexpr bool LDS = decltype(K)::value;
+            // per-lane stack slots only: the wavefronts of a workgroup may trace different scenes
+            size_t sh = LDS ? lds_bytes(trav_block) : 0;
+            int eg = cu_grid((const void *)(k_extend_batch<NL, LDS>), trav_block, sh);
+            int sg = cu_grid((const void *)(k_shadow_batch<NL, LDS>), trav_block, sh);
+            for (int b = 0; b <= max_depth; b++) {
+                int qsel = b & 1;
+                hipLaunchKernelGGL((k_extend_batch<NL, LDS>), dim3(eg), dim3(trav_block), sh, st, B, P, qsel);
+                hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
+                hipLaunchKernelGGL((k_shadow_batch<NL, LDS>), dim3(sg), dim3(trav_block), sh, st, B, P);
+            }
+        });
     });
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
