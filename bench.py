@@ -38,6 +38,14 @@ HBM_PEAK_GBS = 8000.0
 # extend algorithmic bytes: per ray 32 (ray) + 4 (queue index) + 16 (hit), per triangle record 48,
 # per node visit the node record's size (64 B for the quantized 4-wide layout; from scene_info)
 RAY_BYTES, TRI_BYTES = 52, 48
+# HBM traffic of one extend launch of the default workload (room 1024x768, 256 spp), from two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
+# `bench.py --steps 1 --warmup 0 --no-cpu --no-count`: KiB per dispatch averaged over the 117 extend
+# dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
+# FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
+# read inside this process, so the figure is the profiled one; it is reported only for that workload.
+EXTEND_TRAFFIC_KIB = {"fetch": 1235349.8, "fetch_scale": 2.0, "write": 283019.1, "write_scale": 1.0}
+DEFAULT_ROOM = (1024, 768, 256)
 
 
 def copy_calibration(mib=1024):
@@ -167,13 +175,17 @@ def main():
         alg_bytes = rays * RAY_BYTES + c["nodes_visited"] * info["node_bytes"] + c["tris_tested"] * TRI_BYTES
         ext_time_s = (sum(ext_ms) / len(ext_ms)) / 1e3
         achieved = alg_bytes / ext_time_s / 1e9
+        traffic = None
+        if (args.width, args.height, args.spp) == DEFAULT_ROOM:
+            t = EXTEND_TRAFFIC_KIB
+            traffic = round((t["fetch"] * t["fetch_scale"] + t["write"] * t["write_scale"]) * 1024)
         roofline = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_extend",
             "node_bytes": info["node_bytes"],
             "bytes_per_launch": alg_bytes / max(1, c["extend_launches"]),
