@@ -102,6 +102,7 @@ def main():
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--block-size", type=int, default=0, help="room: threads per workgroup (0 = the library's default)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache", "city"], default="room")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache: samples per pixel of the reference")
@@ -147,6 +148,8 @@ def main():
     info = tinypt_amd.scene_info(dev)
     pt = tinypt_amd.PathTracer("hip")
     kw = dict(max_depth=8, rr_depth=5, seed=1)
+    if args.block_size:
+        kw["block_size"] = args.block_size
     print(f"# scene ready in {setup_s:.1f} s: {info}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         pt.render(dev, args.spp, **kw)

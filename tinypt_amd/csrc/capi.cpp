@@ -33,7 +33,7 @@ void launch_finalize(const DRender &, const DPaths &, float *, hipStream_t);
 void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, float *, uint32_t *, bool, int, int,
                           hipStream_t);
 void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
-void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, int, uint64_t, hipStream_t);
+void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, bool, int, int, uint64_t, hipStream_t);
 void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
 void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
 void launch_cache_reset(const DCache &, hipStream_t);
@@ -189,11 +189,14 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.env_cond = (const float *)(base + o_ec);
     S.layout = H.layout;
     const char *pers = std::getenv("TPT_PERSISTENT");
-    // traversal kernels: 0 = one ray per thread, 1 = persistent with per-wave refill,
-    // 2 (default) = persistent + parked leaves
-    S.persistent = 2;
-    if (pers && std::string(pers) == "0") S.persistent = 0;
+    // traversal kernels: 0 (default) = one ray per lane, the wave keeps its 64 consecutive
+    // queue entries until all are done (neighbouring rays fetch the same nodes, so loads
+    // coalesce); 1 = persistent with per-wave refill; 2 = persistent + parked leaves.
+    // Room 1024x768x256: 329 Mpaths/s (0) vs 237 (2): refills mix rays from all over the
+    // queue into a wave and lose that coherence.
+    S.persistent = 0;
     if (pers && std::string(pers) == "1") S.persistent = 1;
+    if (pers && std::string(pers) == "2") S.persistent = 2;
     // traversal stack: the first kLdsStack entries in LDS by default (the private
     // scratch stack spills to memory: ~9x the extend kernel's HBM writes);
     // TPT_LDS_STACK=0 selects the private stack
@@ -202,7 +205,10 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     // persistent traversal workgroups and the LDS node cache of the quantized layout
     // (the top of the tree, in BFS order at the start of the node array)
     S.trav_block = env_int("TPT_TRAV_BLOCK", kTravBlock, 64, 1024) / 64 * 64;
-    S.lds_nodes = H.layout == kNodeBvh4Q ? std::min(H.bfs_prefix, env_int("TPT_LDS_NODES", kLdsNodes, 0, 2048)) : 0;
+    // node cache: 512 nodes for the persistent kernels (one large workgroup per SIMD pair);
+    // none for one-ray-per-lane (its 2048 resident threads per CU need the LDS for their stacks)
+    const int nodes_default = S.persistent ? kLdsNodes : 0;
+    S.lds_nodes = H.layout == kNodeBvh4Q ? std::min(H.bfs_prefix, env_int("TPT_LDS_NODES", nodes_default, 0, 2048)) : 0;
     S.num_nodes = (int32_t)(H.nodes.size() / kNodeFloat4[H.layout]);
     S.root = H.root;
     S.num_tris = (int32_t)(H.tris_bvh.size() / 3);
@@ -379,6 +385,7 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
             stack_bytes = 0;
         }
         h->S.lds_nodes = std::min(h->S.lds_nodes, (max_shared - stack_bytes) / 64);
+        h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
         HIP_OK(hipMalloc(&h->trace_counters, 64));
         *out = h;
         return TPT_OK;
@@ -515,6 +522,8 @@ struct BatchRun {
     std::vector<uint64_t> path_off, pix_off;
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
     bool lds_stack = true; // first traversal-stack entries of each lane in LDS
+    bool lane = true;      // one-ray-per-lane traversal kernels (else persistent)
+    int max_lds = 65536;
     hipStream_t st = nullptr;
     ~BatchRun() {
         if (st) (void)hipStreamSynchronize(st); // nothing queued may still use the buffers
@@ -527,7 +536,8 @@ struct BatchRun {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, num_cu, total_pixels, st);
+        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, lane, max_lds, num_cu, total_pixels,
+                          st);
         HIP_OK(hipStreamEndCapture(st, &g));
         hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -619,6 +629,8 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         run.layout = scenes[0]->S.layout;
         run.trav_block = scenes[0]->S.trav_block;
         run.lds_stack = scenes[0]->S.lds_stack != 0;
+        run.lane = scenes[0]->S.persistent == 0;
+        run.max_lds = scenes[0]->S.max_lds;
         run.num_cu = scenes[0]->cus;
         run.fp64 = p->accum_fp64 ? 1 : 0;
         std::vector<DScene> table(n);

@@ -139,13 +139,14 @@ struct DScene {
     int32_t env_w, env_h;
     int32_t bg_map;
     int32_t layout;     // kNodeBvh2 / kNodeBvh4 / kNodeBvh4Q
-    int32_t persistent; // 1: persistent traversal kernels with per-wave ray refill
+    int32_t persistent; // 0: one ray per lane until the whole wave is done; 1: persistent kernels with
+                        // per-wave ray refill; 2: persistent + parked leaves
     int32_t num_cu;     // compute units of the device (host-side grid sizing)
     int32_t lds_stack;  // 1: first kLdsStack traversal-stack entries in LDS; 0: private (scratch) stack
     int32_t lds_nodes;  // quantized layout: nodes [0, lds_nodes) (the top of the tree in BFS order)
                         // are copied into LDS by each traversal workgroup; 0 = no node cache
     int32_t trav_block; // threads per workgroup of the persistent traversal kernels
-    int32_t pad3;
+    int32_t max_lds;    // bytes of LDS one workgroup may allocate on the device
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

@@ -316,20 +316,34 @@ __device__ __forceinline__ bool trav_step(const DScene &S, V3 o, V3 d, const Ray
 
 // Closest hit (ANY = false) or any hit (ANY = true) with t in [EPS, tmax).
 // Ties on t resolve to the smaller global primitive id.
+// `cache`: nodes [0, cache) are read from the workgroup's LDS copy (stage_top_nodes).
 template <bool ANY, bool COUNT, int NL, bool LDS>
 __device__ __forceinline__ HitRec trace(const DScene &S, V3 o, V3 d, float tmax, const AlphaCtx &ac, uint32_t &nodes,
-                                        uint32_t &tris) {
+                                        uint32_t &tris, int cache) {
     HitRec best{tmax, -1, 0.f, 0.f};
     if (analytic_tests<ANY>(S, o, d, ac, best)) return best;
     if (S.root == -1) return best;
     RayPre rp = ray_pre(o, d);
     TStack<LDS> stk;
-    stk.init(0);
+    stk.init(cache);
     int sp = 0;
     int32_t cur = S.root;
-    while (trav_step<ANY, COUNT, NL>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris, 0)) {
+    while (trav_step<ANY, COUNT, NL>(S, o, d, rp, ac, best, cur, sp, stk, nodes, tris, cache)) {
     }
     return best;
+}
+
+// Copies nodes [0, cache) of the quantized layout (the top of the tree: the
+// node array starts in BFS order) to the front of the workgroup's LDS, where
+// every ray starts.  All threads of the workgroup call it (it synchronises).
+__device__ __forceinline__ void stage_top_nodes(const DScene &S, int cache) {
+    if (cache <= 0) return;
+    lds_f4 *dst = (lds_f4 *)tpt_lds_stack;
+    for (int i = threadIdx.x; i < 4 * cache; i += blockDim.x) {
+        float4 v = S.nodes[i];
+        dst[i] = v4f{v.x, v.y, v.z, v.w};
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------- wave-level compaction helpers
@@ -451,14 +465,7 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
     int32_t leaf = kNoNode; // parked leaf of this lane
     // stage the top of the tree (quantized layout) in LDS: every ray starts there
     const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
-    if (cache > 0) {
-        lds_f4 *dst = (lds_f4 *)tpt_lds_stack;
-        for (int i = threadIdx.x; i < 4 * cache; i += blockDim.x) {
-            float4 v = S.nodes[i];
-            dst[i] = v4f{v.x, v.y, v.z, v.w};
-        }
-        __syncthreads();
-    }
+    stage_top_nodes(S, cache);
     TStack<LDS> stk;
     stk.init(cache);
     const int lane = (int)__lane_id();
@@ -644,8 +651,13 @@ __global__ void k_generate(DScene S, DRender R, DPaths P, int wave_base, int n_p
 }
 
 // ---------------------------------------------------------------- extend
-template <bool COUNT, int NL>
-__global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next) {
+// One ray per lane: a wavefront takes 64 consecutive queue entries and keeps them
+// until all 64 are done.  Neighbouring queue entries are mostly neighbouring
+// pixels, so the lanes of a wave tend to fetch the same nodes and triangles and
+// their loads coalesce.  `cache`: LDS node-cache size (stage_top_nodes).
+template <bool COUNT, int NL, bool LDS>
+__global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next, int cache) {
+    stage_top_nodes(S, cache);
     uint32_t n = P.counters[qsel * kCntLine];
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
@@ -658,7 +670,7 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
         int slot = queue[i];
         float4 o4 = P.org[slot], d4 = P.dir[slot];
         AlphaCtx ac{P.key[slot], P.depth[slot], true};
-        HitRec h = trace<false, COUNT, NL, false>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris);
+        HitRec h = trace<false, COUNT, NL, LDS>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris, cache);
         P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
     }
     if (COUNT) {
@@ -898,8 +910,9 @@ __global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int
 }
 
 // ---------------------------------------------------------------- shadow
-template <bool COUNT, int NL>
-__global__ void k_shadow(DScene S, DPaths P) {
+template <bool COUNT, int NL, bool LDS>
+__global__ void k_shadow(DScene S, DPaths P, int cache) {
+    stage_top_nodes(S, cache);
     uint32_t n = P.counters[kCntShadow];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
@@ -907,7 +920,7 @@ __global__ void k_shadow(DScene S, DPaths P) {
     AlphaCtx ac{0, 0, false};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 o4 = P.shadow_org[i], d4 = P.shadow_dir[i];
-        HitRec h = trace<true, false, NL, false>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris);
+        HitRec h = trace<true, false, NL, LDS>(S, mk4(o4), mk4(d4), o4.w, ac, nodes, tris, cache);
         if (h.gid < 0) {
             int slot = P.shadow_slot[i];
             float4 c = P.color[slot], v = P.shadow_val[i];
@@ -991,15 +1004,16 @@ __global__ void k_finalize(DRender R, DPaths P, float *out) {
 }
 
 // ---------------------------------------------------------------- ray queries (parity / bench)
-template <bool COUNT, int NL>
+template <bool COUNT, int NL, bool LDS>
 __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_t *prim, float *tout,
-                                uint32_t *counters) {
+                                uint32_t *counters, int cache) {
+    stage_top_nodes(S, cache);
     uint32_t nodes = 0, tris = 0;
     AlphaCtx ac{0, 0, false};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<false, COUNT, NL, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        HitRec h = trace<false, COUNT, NL, LDS>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, cache);
         bool hit = h.gid >= 0;
         prim[i] = hit ? h.gid : -1;
         tout[i] = hit ? h.t : kINF;
@@ -1010,14 +1024,15 @@ __global__ void k_trace_closest(DScene S, const float4 *rays, uint64_t n, int32_
     }
 }
 
-template <int NL>
-__global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8_t *occ) {
+template <int NL, bool LDS>
+__global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8_t *occ, int cache) {
+    stage_top_nodes(S, cache);
     uint32_t nodes = 0, tris = 0;
     AlphaCtx ac{0, 0, false};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         float4 a = rays[2 * i], b = rays[2 * i + 1];
         float tmax = b.w < kINF ? b.w : kINF;
-        HitRec h = trace<true, false, NL, false>(S, mk4(a), mk4(b), tmax, ac, nodes, tris);
+        HitRec h = trace<true, false, NL, LDS>(S, mk4(a), mk4(b), tmax, ac, nodes, tris, cache);
         occ[i] = h.gid >= 0 ? 1 : 0;
     }
 }
@@ -1280,6 +1295,65 @@ __global__ TPT_TRAVERSAL_ATTR void k_shadow_batch(DBatch B, DPaths P) {
     }
 }
 
+// One-ray-per-lane batch traversal: lane i of the grid takes queue position i;
+// the scene of position i is the segment holding it (segment bases in
+// `sbase`, counts in `scnt`).  A wavefront runs one pass per distinct scene
+// among its 64 entries (one pass except where segments meet), so the scene is
+// uniform inside a pass and its record is read through scalar loads.
+template <bool ANY, int NL, bool LDS, class Body>
+__device__ __forceinline__ void batch_lane_rays(const DBatch &B, const int32_t *sbase, const int32_t *scnt, Body body) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < (uint32_t)B.cap; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        int s = 0;
+        bool valid = false;
+        if (i < (uint32_t)B.cap) {
+            s = find_seg(sbase, B.n, (int32_t)i);
+            valid = (int32_t)i < sbase[s] + scnt[s];
+        }
+        uint64_t todo = __ballot(valid);
+        while (todo) {
+            int lead = __ffsll((unsigned long long)todo) - 1;
+            int s0 = __builtin_amdgcn_readlane(s, lead);
+            bool mine = valid && s == s0;
+            if (mine) body(scene_at(B, __builtin_amdgcn_readfirstlane(s0)), i);
+            todo &= ~__ballot(mine);
+        }
+    }
+}
+
+template <int NL, bool LDS>
+__global__ void k_extend_batch_lane(DBatch B, DPaths P, int qsel) {
+    const int fb = qsel ? kSegQbBase : kSegQaBase;
+    const int32_t *queue = qsel ? P.queue_b : P.queue_a;
+    batch_lane_rays<false, NL, LDS>(B, B.seg + fb * B.seg_stride, B.seg + (fb + 1) * B.seg_stride,
+                                    [&](const DScene &S, uint32_t i) {
+                                        uint32_t nodes = 0, tris = 0;
+                                        int slot = queue[i];
+                                        float4 o4 = P.org[slot], d4 = P.dir[slot];
+                                        AlphaCtx ac{P.key[slot], P.depth[slot], true};
+                                        HitRec h = trace<false, false, NL, LDS>(S, mk4(o4), mk4(d4), kINF, ac, nodes,
+                                                                                tris, 0);
+                                        P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+                                    });
+}
+
+template <int NL, bool LDS>
+__global__ void k_shadow_batch_lane(DBatch B, DPaths P) {
+    batch_lane_rays<true, NL, LDS>(B, B.seg + kSegShBase * B.seg_stride, B.seg + kSegShCnt * B.seg_stride,
+                                   [&](const DScene &S, uint32_t i) {
+                                       uint32_t nodes = 0, tris = 0;
+                                       AlphaCtx ac{0, 0, false};
+                                       float4 o4 = P.shadow_org[i], d4 = P.shadow_dir[i];
+                                       HitRec h = trace<true, false, NL, LDS>(S, mk4(o4), mk4(d4), o4.w, ac, nodes,
+                                                                              tris, 0);
+                                       if (h.gid < 0) {
+                                           int slot = P.shadow_slot[i];
+                                           float4 c = P.color[slot], v = P.shadow_val[i];
+                                           P.color[slot] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, 0.f);
+                                       }
+                                   });
+}
+
 // One thread per pixel of every scene: adds the samples of this pipeline wave
 // that belong to the pixel, in sample order.
 __global__ void k_accumulate_batch(DBatch B, DPaths P) {
@@ -1358,6 +1432,26 @@ template <class F> static void by_variant(const DScene &S, F &&f) {
     by_bool(S.persistent == 2, [&](auto Q) { by_bool(S.lds_stack != 0, [&](auto L) { f(Q, L); }); });
 }
 // Dynamic LDS of a persistent traversal launch: cached nodes + the LDS stack.
+// LDS plan of a one-ray-per-lane launch of `block` threads: the stack's first
+// kLdsStack entries per lane when S.lds_stack is on and they fit, then as much
+// of the node cache (quantized layout) as the rest of the workgroup's LDS holds.
+struct LaneLds {
+    bool stack;
+    int cache;
+    size_t bytes;
+};
+static LaneLds lane_lds(const DScene &S, int nl, int block) {
+    LaneLds r{S.lds_stack != 0, nl == kNodeBvh4Q ? S.lds_nodes : 0, 0};
+    size_t stack = r.stack ? lds_bytes(block) : 0;
+    if (stack > (size_t)S.max_lds) {
+        r.stack = false;
+        stack = 0;
+    }
+    r.cache = std::max(0, std::min(r.cache, (int)(((size_t)S.max_lds - stack) / 64)));
+    r.bytes = stack + (size_t)r.cache * 64;
+    return r;
+}
+
 static size_t trav_shared(const DScene &S, int nl, bool lds_stack) {
     size_t nodes = nl == kNodeBvh4Q ? (size_t)S.lds_nodes * 64 : 0;
     return nodes + (lds_stack ? lds_bytes(S.trav_block) : 0);
@@ -1370,7 +1464,11 @@ void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int q
             constexpr int NL = decltype(L)::value;
             constexpr bool CNT = decltype(C)::value;
             if (!S.persistent) {
-                hipLaunchKernelGGL((k_extend<CNT, NL>), dim3(grid), dim3(block), 0, st, S, P, queue, qsel, nsel);
+                LaneLds m = lane_lds(S, NL, block);
+                by_bool(m.stack, [&](auto D) {
+                    hipLaunchKernelGGL((k_extend<CNT, NL, decltype(D)::value>), dim3(grid), dim3(block), m.bytes, st, S,
+                                       P, queue, qsel, nsel, m.cache);
+                });
                 return;
             }
             by_variant(S, [&](auto Q, auto D) {
@@ -1397,7 +1495,11 @@ void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool c
             constexpr int NL = decltype(L)::value;
             constexpr bool CNT = decltype(C)::value;
             if (!S.persistent) {
-                hipLaunchKernelGGL((k_shadow<CNT, NL>), dim3(grid), dim3(block), 0, st, S, P);
+                LaneLds m = lane_lds(S, NL, block);
+                by_bool(m.stack, [&](auto D) {
+                    hipLaunchKernelGGL((k_shadow<CNT, NL, decltype(D)::value>), dim3(grid), dim3(block), m.bytes, st,
+                                       S, P, m.cache);
+                });
                 return;
             }
             by_variant(S, [&](auto Q, auto D) {
@@ -1425,8 +1527,11 @@ void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32
             constexpr int NL = decltype(L)::value;
             constexpr bool CNT = decltype(C)::value;
             if (!S.persistent) {
-                hipLaunchKernelGGL((k_trace_closest<CNT, NL>), dim3(grid), dim3(block), 0, st, S, rays, n, prim, t,
-                                   counters);
+                LaneLds m = lane_lds(S, NL, block);
+                by_bool(m.stack, [&](auto D) {
+                    hipLaunchKernelGGL((k_trace_closest<CNT, NL, decltype(D)::value>), dim3(grid), dim3(block),
+                                       m.bytes, st, S, rays, n, prim, t, counters, m.cache);
+                });
                 return;
             }
             SrcRays src{rays, prim, t, nullptr};
@@ -1446,7 +1551,11 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
     by_layout(S.layout, [&](auto L) {
         constexpr int NL = decltype(L)::value;
         if (!S.persistent) {
-            hipLaunchKernelGGL((k_trace_occluded<NL>), dim3(grid), dim3(block), 0, st, S, rays, n, occ);
+            LaneLds m = lane_lds(S, NL, block);
+            by_bool(m.stack, [&](auto D) {
+                hipLaunchKernelGGL((k_trace_occluded<NL, decltype(D)::value>), dim3(grid), dim3(block), m.bytes, st, S,
+                                   rays, n, occ, m.cache);
+            });
             return;
         }
         SrcRays src{rays, nullptr, nullptr, occ};
@@ -1466,7 +1575,7 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
 // device memory, so the sequence can be captured once in a hipGraph and replayed
 // ceil(total / cap) times.
 void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layout, int block, int trav_block,
-                       bool lds_stack, int num_cu, uint64_t total_pixels, hipStream_t st) {
+                       bool lds_stack, bool lane, int max_lds, int num_cu, uint64_t total_pixels, hipStream_t st) {
     auto cu_grid = [&](const void *k, int tb, size_t sh) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, tb, sh) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -1476,6 +1585,28 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
     int shade_grid = std::min(gen_grid, num_cu * 2048 / block);
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
+    if (lane) {
+        // one ray per lane: the traversal kernels run with the pipeline's block size
+        bool stk = lds_stack && lds_bytes(block) <= (size_t)max_lds;
+        size_t sh = stk ? lds_bytes(block) : 0;
+        by_layout(layout, [&](auto L) {
+            by_bool(stk, [&](auto K) {
+                constexpr int NL = decltype(L)::value;
+                constexpr bool LDS = decltype(K)::value;
+                for (int b = 0; b <= max_depth; b++) {
+                    int qsel = b & 1;
+                    hipLaunchKernelGGL((k_extend_batch_lane<NL, LDS>), dim3(shade_grid), dim3(block), sh, st, B, P,
+                                       qsel);
+                    hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                    hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
+                    hipLaunchKernelGGL((k_shadow_batch_lane<NL, LDS>), dim3(shade_grid), dim3(block), sh, st, B, P);
+                }
+            });
+        });
+        hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
+        hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
+        return;
+    }
     by_layout(layout, [&](auto L) {
         by_bool(lds_stack, [&](auto K) {
             constexpr int NL = decltype(L)::value;
