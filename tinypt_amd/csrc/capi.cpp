@@ -24,16 +24,17 @@
 
 namespace tpt {
 void launch_generate(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, hipStream_t);
-void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, hipStream_t);
+void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, uint64_t, hipStream_t);
 void launch_shade(const DScene &, const DRender &, const DPaths &, const int32_t *, int32_t *, int, int, int, int,
                   hipStream_t, const DCache *);
-void launch_shadow(const DScene &, const DPaths &, int, int, bool, hipStream_t);
+void launch_shadow(const DScene &, const DPaths &, int, int, bool, uint64_t, hipStream_t);
 void launch_accumulate(const DRender &, const DPaths &, int, int, int, hipStream_t);
 void launch_finalize(const DRender &, const DPaths &, float *, hipStream_t);
 void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, float *, uint32_t *, bool, int, int,
                           hipStream_t);
 void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
-void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, bool, int, int, uint64_t, hipStream_t);
+void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, int, int, int, int, uint64_t,
+                       hipStream_t);
 void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
 void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
 void launch_cache_reset(const DCache &, hipStream_t);
@@ -189,14 +190,18 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.env_cond = (const float *)(base + o_ec);
     S.layout = H.layout;
     const char *pers = std::getenv("TPT_PERSISTENT");
-    // traversal kernels: 0 (default) = one ray per lane, the wave keeps its 64 consecutive
-    // queue entries until all are done (neighbouring rays fetch the same nodes, so loads
-    // coalesce); 1 = persistent with per-wave refill; 2 = persistent + parked leaves.
-    // Room 1024x768x256: 329 Mpaths/s (0) vs 237 (2): refills mix rays from all over the
-    // queue into a wave and lose that coherence.
-    S.persistent = 0;
-    if (pers && std::string(pers) == "1") S.persistent = 1;
-    if (pers && std::string(pers) == "2") S.persistent = 2;
+    // traversal kernels (room 1024x768x256, Mpaths/s):
+    //   0 one ray per lane; a wave keeps its 64 consecutive queue entries until all are done: 331
+    //   1 persistent, idle lanes refilled from one launch-wide counter: 237
+    //   2 1 + parked leaves: 237
+    //   3 each wave refills from its own chunk of consecutive queue entries (no shared counter): 414
+    //   4 (default) 3 + parked leaves: 434
+    // Neighbouring queue entries are mostly neighbouring pixels, so a wave whose rays stay
+    // neighbours fetches the same nodes; the launch-wide counter both mixes rays from all over
+    // the queue and serialises every refill at one L2 channel (L2 read latency 700 vs 233 cycles).
+    S.persistent = 4;
+    if (pers && std::string(pers).size() == 1 && pers[0] >= '0' && pers[0] <= '4') S.persistent = pers[0] - '0';
+    S.chunk = env_int("TPT_CHUNK", 384, 64, 1 << 20) / 64 * 64;
     // traversal stack: the first kLdsStack entries in LDS by default (the private
     // scratch stack spills to memory: ~9x the extend kernel's HBM writes);
     // TPT_LDS_STACK=0 selects the private stack
@@ -207,7 +212,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.trav_block = env_int("TPT_TRAV_BLOCK", kTravBlock, 64, 1024) / 64 * 64;
     // node cache: 512 nodes for the persistent kernels (one large workgroup per SIMD pair);
     // none for one-ray-per-lane (its 2048 resident threads per CU need the LDS for their stacks)
-    const int nodes_default = S.persistent ? kLdsNodes : 0;
+    const int nodes_default = (S.persistent == 1 || S.persistent == 2) ? kLdsNodes : 0;
     S.lds_nodes = H.layout == kNodeBvh4Q ? std::min(H.bfs_prefix, env_int("TPT_LDS_NODES", nodes_default, 0, 2048)) : 0;
     S.num_nodes = (int32_t)(H.nodes.size() / kNodeFloat4[H.layout]);
     S.root = H.root;
@@ -454,11 +459,11 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
                 int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
                 int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
                 int e0 = timing ? ev() : -1;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, st);
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, st);
                 int e1 = timing ? ev() : -1;
                 launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, nullptr);
                 int e2 = timing ? ev() : -1;
-                launch_shadow(S, P, grid, block, count, st);
+                launch_shadow(S, P, grid, block, count, (uint64_t)n, st);
                 int e3 = timing ? ev() : -1;
                 if (timing) {
                     ext_ev.push_back({e0, e1});
@@ -522,7 +527,8 @@ struct BatchRun {
     std::vector<uint64_t> path_off, pix_off;
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
     bool lds_stack = true; // first traversal-stack entries of each lane in LDS
-    bool lane = true;      // one-ray-per-lane traversal kernels (else persistent)
+    int mode = 4;          // traversal kernels (DScene::persistent)
+    int chunk = 384;       // queue entries per wave of the chunked kernels
     int max_lds = 65536;
     hipStream_t st = nullptr;
     ~BatchRun() {
@@ -536,8 +542,8 @@ struct BatchRun {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, lane, max_lds, num_cu, total_pixels,
-                          st);
+        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, mode, chunk, max_lds, num_cu,
+                          total_pixels, st);
         HIP_OK(hipStreamEndCapture(st, &g));
         hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -629,7 +635,8 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         run.layout = scenes[0]->S.layout;
         run.trav_block = scenes[0]->S.trav_block;
         run.lds_stack = scenes[0]->S.lds_stack != 0;
-        run.lane = scenes[0]->S.persistent == 0;
+        run.mode = scenes[0]->S.persistent;
+        run.chunk = scenes[0]->S.chunk;
         run.max_lds = scenes[0]->S.max_lds;
         run.num_cu = scenes[0]->cus;
         run.fp64 = p->accum_fp64 ? 1 : 0;
@@ -686,12 +693,12 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.state = (uint64_t *)(m + o_st);
         B.n = n;
         B.seg_stride = seg_stride_for(n);
-        // candidate capacities: 2^18 .. 2^22 path slots, never (much) more than the batch needs
+        // candidate capacities: 2^18 .. 2^24 path slots, never (much) more than the batch needs
         auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
         std::vector<int> caps;
         if (p->queue_capacity > 0) caps.push_back(round_cap((uint64_t)p->queue_capacity));
         else
-            for (int e = 18; e <= 22; e++) {
+            for (int e = 18; e <= 24; e++) {
                 int c = round_cap(1ull << e);
                 if (caps.empty() || caps.back() != c) caps.push_back(c);
             }
@@ -932,9 +939,9 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
             for (int b = 0; b <= R.max_depth; b++) {
                 int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
                 int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, false, st);
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, false, (uint64_t)pixels, st);
                 launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, use_cache ? &C : nullptr);
-                launch_shadow(S, P, grid, block, false, st);
+                launch_shadow(S, P, grid, block, false, (uint64_t)pixels, st);
                 cur = 1 - cur;
             }
             if (use_cache) {

@@ -140,13 +140,16 @@ struct DScene {
     int32_t bg_map;
     int32_t layout;     // kNodeBvh2 / kNodeBvh4 / kNodeBvh4Q
     int32_t persistent; // 0: one ray per lane until the whole wave is done; 1: persistent kernels with
-                        // per-wave ray refill; 2: persistent + parked leaves
+                        // per-wave ray refill; 2: persistent + parked leaves; 3: each wave traces its own
+                        // `chunk` consecutive queue entries, refilling idle lanes from them; 4: 3 + parked leaves
     int32_t num_cu;     // compute units of the device (host-side grid sizing)
     int32_t lds_stack;  // 1: first kLdsStack traversal-stack entries in LDS; 0: private (scratch) stack
     int32_t lds_nodes;  // quantized layout: nodes [0, lds_nodes) (the top of the tree in BFS order)
                         // are copied into LDS by each traversal workgroup; 0 = no node cache
     int32_t trav_block; // threads per workgroup of the persistent traversal kernels
     int32_t max_lds;    // bytes of LDS one workgroup may allocate on the device
+    int32_t chunk;      // queue entries per wave of the chunked traversal kernels (persistent 3, 4)
+    int32_t pad4;
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;
@@ -239,6 +242,8 @@ enum {
     kSegQaBase = 0, kSegQaCnt, kSegQbBase, kSegQbCnt, // ping-pong ray queues
     kSegFetch,                                        // extend refill counter
     kSegShBase, kSegShCnt, kSegShFetch,               // shadow queue
+    kSegRayChunk, kSegShChunk,                        // chunked traversal: exclusive prefix of chunks per
+                                                      // scene (ray / shadow queue), total at index n
     kSegFields
 };
 struct DBatch {
@@ -252,7 +257,7 @@ struct DBatch {
     uint64_t *state;            // [0] g0 of the current pipeline wave, [1] total paths
     int32_t n;
     int32_t cap;
-    int32_t seg_stride;         // n rounded up to kSegAlign: every field starts on its own cache lines
+    int32_t seg_stride;         // n + 1 rounded up to kSegAlign: every field starts on its own cache lines
     int32_t pad_;
 };
 // Segment fields are kept on separate 256-B lines: the counters that a kernel
@@ -260,6 +265,6 @@ struct DBatch {
 // that every lane of the same kernel reads, and those reads would queue behind
 // the atomics at one L2 channel.
 static constexpr int kSegAlign = 64;
-inline int seg_stride_for(int n) { return (n + kSegAlign - 1) / kSegAlign * kSegAlign; }
+inline int seg_stride_for(int n) { return (n + 1 + kSegAlign - 1) / kSegAlign * kSegAlign; }
 
 } // namespace tpt

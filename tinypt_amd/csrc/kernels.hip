@@ -459,13 +459,30 @@ __device__ __forceinline__ void inner_visit(const DScene &S, const RayPre &rp, f
     cur = sp > 0 ? stk.get(--sp) : kNoNode;
 }
 
-template <bool ANY, bool COUNT, int NL, bool POSTPONE, bool LDS, class Src>
-__device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src, uint32_t n, uint32_t *fetch,
-                                                 uint32_t &nodes, uint32_t &tris) {
+// Ray index sources of the refilling traversal loop.  take(k) returns the first
+// of k consecutive ray indices for the wave's idle lanes (wave-uniform).
+struct FetchGlobal { // one counter shared by every wave of the launch
+    uint32_t *ctr;
+    __device__ __forceinline__ uint32_t take(uint32_t k, int lane, int leader) {
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(ctr, k);
+        return __shfl(base, leader);
+    }
+};
+struct FetchLocal { // the wave's own range of the queue: no atomics, rays stay neighbours
+    uint32_t cursor;
+    __device__ __forceinline__ uint32_t take(uint32_t k, int, int) {
+        uint32_t b = cursor;
+        cursor += k;
+        return b;
+    }
+};
+
+// Rays [.., n) from `fetch`; `cache`: LDS node-cache size (staged by the caller).
+template <bool ANY, bool COUNT, int NL, bool POSTPONE, bool LDS, class Src, class Fetch>
+__device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src, uint32_t n, Fetch &fetch,
+                                                 uint32_t &nodes, uint32_t &tris, int cache) {
     int32_t leaf = kNoNode; // parked leaf of this lane
-    // stage the top of the tree (quantized layout) in LDS: every ray starts there
-    const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
-    stage_top_nodes(S, cache);
     TStack<LDS> stk;
     stk.init(cache);
     const int lane = (int)__lane_id();
@@ -485,9 +502,7 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
         int nneed = __popcll(need);
         if (nneed > 0 && (nneed >= kRefillLanes || act == 0)) {
             int leader = __ffsll((unsigned long long)need) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(fetch, (uint32_t)nneed);
-            base = __shfl(base, leader);
+            uint32_t base = fetch.take((uint32_t)nneed, lane, leader);
             if ((uint64_t)base + (uint64_t)nneed >= (uint64_t)n) exhausted = true;
             if ((need >> lane) & 1ull) {
                 uint32_t idx = base + (uint32_t)__popcll(need & lower);
@@ -575,10 +590,69 @@ __global__ TPT_TRAVERSAL_ATTR void k_extend_pt(DScene S, DPaths P, const int32_t
     }
     uint32_t nodes = 0, tris = 0;
     SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
-    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, n, P.counters + kCntFetch, nodes, tris);
+    const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
+    stage_top_nodes(S, cache);
+    FetchGlobal f{P.counters + kCntFetch};
+    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, n, f, nodes, tris, cache);
     if (COUNT) {
         wave_count(nodes, P.counters + kCntTraffic);
         wave_count(tris, P.counters + kCntTraffic + 2);
+    }
+}
+
+// Chunked traversal: wave w owns queue entries [w * chunk, (w + 1) * chunk) and
+// refills its idle lanes from them (FetchLocal), so lanes stay busy while the
+// wave's rays stay neighbours in the queue; no shared counter.
+template <bool COUNT, int NL, bool POST, bool LDS>
+__global__ void k_extend_chunk(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next, int chunk,
+                               int cache) {
+    stage_top_nodes(S, cache);
+    uint32_t n = P.counters[qsel * kCntLine];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.counters[nsel_next * kCntLine] = 0u; // next ray queue
+        P.counters[kCntShadow] = 0u;
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 4), (unsigned long long)n);
+    }
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    if (begin >= n) return; // wave-uniform
+    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t nodes = 0, tris = 0;
+    SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
+    FetchLocal f{(uint32_t)begin};
+    persistent_trace<false, COUNT, NL, POST, LDS>(S, src, end, f, nodes, tris, cache);
+    if (COUNT) {
+        wave_count(nodes, P.counters + kCntTraffic);
+        wave_count(tris, P.counters + kCntTraffic + 2);
+    }
+}
+
+template <bool COUNT, int NL, bool POST, bool LDS>
+__global__ void k_shadow_chunk(DScene S, DPaths P, int chunk, int cache) {
+    stage_top_nodes(S, cache);
+    uint32_t n = P.counters[kCntShadow];
+    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    if (begin >= n) return;
+    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t nodes = 0, tris = 0;
+    SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
+    FetchLocal f{(uint32_t)begin};
+    persistent_trace<true, false, NL, POST, LDS>(S, src, end, f, nodes, tris, cache);
+}
+
+template <bool ANY, bool COUNT, int NL, bool POST, bool LDS>
+__global__ void k_trace_chunk(DScene S, SrcRays src, uint32_t n, uint32_t *counters, int chunk, int cache) {
+    stage_top_nodes(S, cache);
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    if (begin >= n) return;
+    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t nodes = 0, tris = 0;
+    FetchLocal f{(uint32_t)begin};
+    persistent_trace<ANY, COUNT, NL, POST, LDS>(S, src, end, f, nodes, tris, cache);
+    if (COUNT) {
+        wave_count(nodes, counters + 0);
+        wave_count(tris, counters + 2);
     }
 }
 
@@ -589,13 +663,19 @@ __global__ TPT_TRAVERSAL_ATTR void k_shadow_pt(DScene S, DPaths P) {
         atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
     uint32_t nodes = 0, tris = 0;
     SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
-    persistent_trace<true, false, NL, POST, LDS>(S, src, n, P.counters + kCntShFetch, nodes, tris);
+    const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
+    stage_top_nodes(S, cache);
+    FetchGlobal f{P.counters + kCntShFetch};
+    persistent_trace<true, false, NL, POST, LDS>(S, src, n, f, nodes, tris, cache);
 }
 
 template <bool ANY, bool COUNT, int NL, bool POST, bool LDS>
 __global__ TPT_TRAVERSAL_ATTR void k_trace_pt(DScene S, SrcRays src, uint32_t n, uint32_t *counters) {
     uint32_t nodes = 0, tris = 0;
-    persistent_trace<ANY, COUNT, NL, POST, LDS>(S, src, n, counters + 8, nodes, tris);
+    const int cache = NL == kNodeBvh4Q ? S.lds_nodes : 0;
+    stage_top_nodes(S, cache);
+    FetchGlobal f{counters + 8};
+    persistent_trace<ANY, COUNT, NL, POST, LDS>(S, src, n, f, nodes, tris, cache);
     if (COUNT) {
         wave_count(nodes, counters + 0);
         wave_count(tris, counters + 2);
@@ -1200,8 +1280,8 @@ __global__ TPT_TRAVERSAL_ATTR void k_extend_batch(DBatch B, DPaths P, int qsel) 
         if (cnt == 0) continue;
         int base = TPT_SEG(B, fb, s);
         SrcQueue src{queue + base, P.org, P.dir, P.key, P.depth, P.hit + base};
-        persistent_trace<false, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt,
-                                                      (uint32_t *)&TPT_SEG(B, kSegFetch, s), nodes, tris);
+        FetchGlobal f{(uint32_t *)&TPT_SEG(B, kSegFetch, s)};
+        persistent_trace<false, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt, f, nodes, tris, 0);
     }
 }
 
@@ -1290,8 +1370,8 @@ __global__ TPT_TRAVERSAL_ATTR void k_shadow_batch(DBatch B, DPaths P) {
         if (cnt == 0) continue;
         int base = TPT_SEG(B, kSegShBase, s);
         SrcShadow src{P.shadow_org + base, P.shadow_dir + base, P.shadow_val + base, P.shadow_slot + base, P.color};
-        persistent_trace<true, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt,
-                                                     (uint32_t *)&TPT_SEG(B, kSegShFetch, s), nodes, tris);
+        FetchGlobal f{(uint32_t *)&TPT_SEG(B, kSegShFetch, s)};
+        persistent_trace<true, false, NL, true, LDS>(scene_at(B, s), src, (uint32_t)cnt, f, nodes, tris, 0);
     }
 }
 
@@ -1352,6 +1432,59 @@ __global__ void k_shadow_batch_lane(DBatch B, DPaths P) {
                                            P.color[slot] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, 0.f);
                                        }
                                    });
+}
+
+// Chunks of the scenes' segments for the chunked traversal kernels: out[s] =
+// exclusive prefix over scenes of ceil(count_s / chunk), out[n] = the total.
+// One workgroup of 1024 threads, at most 1024 scenes.
+__global__ void k_batch_chunks(DBatch B, int fbase, int fcnt, int fout, int chunk) {
+    __shared__ int32_t buf[1024];
+    int t = threadIdx.x;
+    int c = t < B.n ? (TPT_SEG(B, fcnt, t) + chunk - 1) / chunk : 0;
+    buf[t] = c;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int v = t >= off ? buf[t - off] : 0;
+        __syncthreads();
+        buf[t] += v;
+        __syncthreads();
+    }
+    if (t < B.n) TPT_SEG(B, fout, t) = buf[t] - c;
+    if (t == B.n - 1) TPT_SEG(B, fout, B.n) = buf[t];
+}
+
+// Chunked batch traversal: wave w takes chunk w of the concatenated per-scene
+// chunk lists, i.e. `chunk` consecutive entries of one scene's segment, and
+// refills its idle lanes from them; the scene is uniform across the wave.
+template <bool ANY, int NL, bool POST, bool LDS, class MakeSrc>
+__device__ __forceinline__ void batch_chunk_rays(const DBatch &B, int fbase, int fcnt, int fchunk, int chunk,
+                                                 MakeSrc make_src) {
+    const int32_t *pre = B.seg + fchunk * B.seg_stride;
+    int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (w >= pre[B.n]) return;
+    int s = __builtin_amdgcn_readfirstlane(find_seg(pre, B.n, w));
+    uint32_t base = (uint32_t)TPT_SEG(B, fbase, s), cnt = (uint32_t)TPT_SEG(B, fcnt, s);
+    uint32_t begin = base + (uint32_t)(w - pre[s]) * (uint32_t)chunk;
+    uint32_t end = min(begin + (uint32_t)chunk, base + cnt);
+    uint32_t nodes = 0, tris = 0;
+    FetchLocal f{begin};
+    persistent_trace<ANY, false, NL, POST, LDS>(scene_at(B, s), make_src(), end, f, nodes, tris, 0);
+}
+
+template <int NL, bool POST, bool LDS>
+__global__ void k_extend_batch_chunk(DBatch B, DPaths P, int qsel, int chunk) {
+    const int fb = qsel ? kSegQbBase : kSegQaBase;
+    const int32_t *queue = qsel ? P.queue_b : P.queue_a;
+    batch_chunk_rays<false, NL, POST, LDS>(B, fb, fb + 1, kSegRayChunk, chunk, [&]() {
+        return SrcQueue{queue, P.org, P.dir, P.key, P.depth, P.hit};
+    });
+}
+
+template <int NL, bool POST, bool LDS>
+__global__ void k_shadow_batch_chunk(DBatch B, DPaths P, int chunk) {
+    batch_chunk_rays<true, NL, POST, LDS>(B, kSegShBase, kSegShCnt, kSegShChunk, chunk, [&]() {
+        return SrcShadow{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
+    });
 }
 
 // One thread per pixel of every scene: adds the samples of this pipeline wave
@@ -1457,8 +1590,23 @@ static size_t trav_shared(const DScene &S, int nl, bool lds_stack) {
     return nodes + (lds_stack ? lds_bytes(S.trav_block) : 0);
 }
 
+// Chunk size of a launch over at most `max_rays` entries: the configured size,
+// reduced (down to one wave's worth) so that the launch still has 16 waves per
+// CU when the rays allow it; small renders would otherwise leave CUs idle.
+static int chunk_for(uint64_t max_rays, int chunk, int num_cu) {
+    uint64_t c = max_rays / ((uint64_t)num_cu * 16) / 64 * 64;
+    return (int)std::max<uint64_t>(64, std::min<uint64_t>((uint64_t)chunk, c));
+}
+
+// Workgroups covering `max_rays` queue entries at `chunk` entries per wave.
+static int chunk_grid(uint64_t max_rays, int chunk, int block) {
+    uint64_t waves = (max_rays + (uint64_t)chunk - 1) / (uint64_t)chunk;
+    uint64_t per_block = (uint64_t)block / 64;
+    return (int)std::max<uint64_t>(1, (waves + per_block - 1) / per_block);
+}
+
 void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int qsel, int nsel, int grid, int block,
-                   bool count, hipStream_t st) {
+                   bool count, uint64_t max_rays, hipStream_t st) {
     by_layout(S.layout, [&](auto L) {
         by_bool(count, [&](auto C) {
             constexpr int NL = decltype(L)::value;
@@ -1468,6 +1616,18 @@ void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int q
                 by_bool(m.stack, [&](auto D) {
                     hipLaunchKernelGGL((k_extend<CNT, NL, decltype(D)::value>), dim3(grid), dim3(block), m.bytes, st, S,
                                        P, queue, qsel, nsel, m.cache);
+                });
+                return;
+            }
+            if (S.persistent >= 3) {
+                LaneLds m = lane_lds(S, NL, block);
+                int ck = chunk_for(max_rays, S.chunk, S.num_cu);
+                int g = chunk_grid(max_rays, ck, block);
+                by_bool(S.persistent == 4, [&](auto Q) {
+                    by_bool(m.stack, [&](auto D) {
+                        hipLaunchKernelGGL((k_extend_chunk<CNT, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
+                                           dim3(block), m.bytes, st, S, P, queue, qsel, nsel, ck, m.cache);
+                    });
                 });
                 return;
             }
@@ -1489,7 +1649,8 @@ void launch_shade(const DScene &S, const DRender &R, const DPaths &P, const int3
     else
         hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(block), 0, st, S, R, P, queue, next, qsel, nsel, DCache{});
 }
-void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, hipStream_t st) {
+void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool count, uint64_t max_rays,
+                   hipStream_t st) {
     by_layout(S.layout, [&](auto L) {
         by_bool(count, [&](auto C) {
             constexpr int NL = decltype(L)::value;
@@ -1499,6 +1660,18 @@ void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool c
                 by_bool(m.stack, [&](auto D) {
                     hipLaunchKernelGGL((k_shadow<CNT, NL, decltype(D)::value>), dim3(grid), dim3(block), m.bytes, st,
                                        S, P, m.cache);
+                });
+                return;
+            }
+            if (S.persistent >= 3) {
+                LaneLds m = lane_lds(S, NL, block);
+                int ck = chunk_for(max_rays, S.chunk, S.num_cu);
+                int g = chunk_grid(max_rays, ck, block);
+                by_bool(S.persistent == 4, [&](auto Q) {
+                    by_bool(m.stack, [&](auto D) {
+                        hipLaunchKernelGGL((k_shadow_chunk<CNT, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
+                                           dim3(block), m.bytes, st, S, P, ck, m.cache);
+                    });
                 });
                 return;
             }
@@ -1535,6 +1708,19 @@ void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32
                 return;
             }
             SrcRays src{rays, prim, t, nullptr};
+            if (S.persistent >= 3) {
+                LaneLds m = lane_lds(S, NL, block);
+                int ck = chunk_for(n, S.chunk, S.num_cu);
+                int g = chunk_grid(n, ck, block);
+                by_bool(S.persistent == 4, [&](auto Q) {
+                    by_bool(m.stack, [&](auto D) {
+                        hipLaunchKernelGGL((k_trace_chunk<false, CNT, NL, decltype(Q)::value, decltype(D)::value>),
+                                           dim3(g), dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, ck,
+                                           m.cache);
+                    });
+                });
+                return;
+            }
             by_variant(S, [&](auto Q, auto D) {
                 constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
                 const int tb = S.trav_block;
@@ -1559,6 +1745,18 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
             return;
         }
         SrcRays src{rays, nullptr, nullptr, occ};
+        if (S.persistent >= 3) {
+            LaneLds m = lane_lds(S, NL, block);
+            int ck = chunk_for(n, S.chunk, S.num_cu);
+            int g = chunk_grid(n, ck, block);
+            by_bool(S.persistent == 4, [&](auto Q) {
+                by_bool(m.stack, [&](auto D) {
+                    hipLaunchKernelGGL((k_trace_chunk<true, false, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
+                                       dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, ck, m.cache);
+                });
+            });
+            return;
+        }
         by_variant(S, [&](auto Q, auto D) {
             constexpr bool POST = decltype(Q)::value, LDS = decltype(D)::value;
             const int tb = S.trav_block;
@@ -1575,7 +1773,9 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
 // device memory, so the sequence can be captured once in a hipGraph and replayed
 // ceil(total / cap) times.
 void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layout, int block, int trav_block,
-                       bool lds_stack, bool lane, int max_lds, int num_cu, uint64_t total_pixels, hipStream_t st) {
+                       bool lds_stack, int mode, int chunk, int max_lds, int num_cu, uint64_t total_pixels,
+                       hipStream_t st) {
+    const bool lane = mode == 0;
     auto cu_grid = [&](const void *k, int tb, size_t sh) {
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, tb, sh) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -1585,6 +1785,38 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
     int shade_grid = std::min(gen_grid, num_cu * 2048 / block);
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, st, B);
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
+    if (mode >= 3) {
+        // chunked traversal with the pipeline's block size: at most cap / chunk + n chunks
+        chunk = chunk_for((uint64_t)B.cap, chunk, num_cu);
+        bool stk = lds_stack && lds_bytes(block) <= (size_t)max_lds;
+        size_t sh = stk ? lds_bytes(block) : 0;
+        int cg = chunk_grid((uint64_t)B.cap + (uint64_t)B.n * (uint64_t)chunk, chunk, block);
+        by_layout(layout, [&](auto L) {
+            by_bool(mode == 4, [&](auto Q) {
+                by_bool(stk, [&](auto K) {
+                    constexpr int NL = decltype(L)::value;
+                    constexpr bool POST = decltype(Q)::value, LDS = decltype(K)::value;
+                    for (int b = 0; b <= max_depth; b++) {
+                        int qsel = b & 1;
+                        int fb = qsel ? kSegQbBase : kSegQaBase;
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, fb, fb + 1, (int)kSegRayChunk,
+                                           chunk);
+                        hipLaunchKernelGGL((k_extend_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B, P,
+                                           qsel, chunk);
+                        hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                        hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, (int)kSegShBase,
+                                           (int)kSegShCnt, (int)kSegShChunk, chunk);
+                        hipLaunchKernelGGL((k_shadow_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B, P,
+                                           chunk);
+                    }
+                });
+            });
+        });
+        hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
+        hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
+        return;
+    }
     if (lane) {
         // one ray per lane: the traversal kernels run with the pipeline's block size
         bool stk = lds_stack && lds_bytes(block) <= (size_t)max_lds;
