@@ -243,7 +243,8 @@ enum {
     kSegFetch,                                        // extend refill counter
     kSegShBase, kSegShCnt, kSegShFetch,               // shadow queue
     kSegRayChunk, kSegShChunk,                        // chunked traversal: exclusive prefix of chunks per
-                                                      // scene (ray / shadow queue), total at index n
+                                                      // scene (ray / shadow queue), total at index n, the
+                                                      // chunk size at n + 1
     kSegFields
 };
 struct DBatch {
@@ -257,7 +258,7 @@ struct DBatch {
     uint64_t *state;            // [0] g0 of the current pipeline wave, [1] total paths
     int32_t n;
     int32_t cap;
-    int32_t seg_stride;         // n + 1 rounded up to kSegAlign: every field starts on its own cache lines
+    int32_t seg_stride;         // n + 2 rounded up to kSegAlign: every field starts on its own cache lines
     int32_t pad_;
 };
 // Segment fields are kept on separate 256-B lines: the counters that a kernel
@@ -265,6 +266,6 @@ struct DBatch {
 // that every lane of the same kernel reads, and those reads would queue behind
 // the atomics at one L2 channel.
 static constexpr int kSegAlign = 64;
-inline int seg_stride_for(int n) { return (n + 1 + kSegAlign - 1) / kSegAlign * kSegAlign; }
+inline int seg_stride_for(int n) { return (n + 2 + kSegAlign - 1) / kSegAlign * kSegAlign; }
 
 } // namespace tpt

@@ -368,9 +368,10 @@ __device__ __forceinline__ void wave_count(uint32_t v32, uint32_t *counter) {
 
 // ---------------------------------------------------------------- persistent traversal
 // Each wave keeps its 64 lanes busy: a lane that finishes its ray takes a new
-// one from a global counter as soon as kRefillLanes lanes of the wave are idle
-// (one atomic per refill for the whole wave), instead of idling until the
-// slowest ray of the wave is done.  Ray sources / sinks:
+// one as soon as kRefillLanes lanes of the wave are idle, instead of idling
+// until the slowest ray of the wave is done.  New rays come from the wave's own
+// chunk of the queue (FetchLocal, the default kernels) or from a counter shared
+// by the launch (FetchGlobal, one atomic per refill).  Ray sources / sinks:
 // Streamed path-state records (each read or written once per launch) use
 // non-temporal loads / stores, so they do not evict BVH nodes and triangles
 // from the L2 while the traversal runs.
@@ -441,7 +442,10 @@ struct SrcRays { // ray-query API: (org, tmin, dir, tmax) records -> prim id + t
     }
 };
 
-static constexpr int kRefillLanes = 16;
+// Refill once 24 of the 64 lanes are idle; test parked leaves once 16 lanes hold
+// one (room 1024x768x256, chunked kernels: refill at 8 / 16 / 24 / 32 idle lanes
+// gives 401 / 433 / 444 / 440 Mpaths/s; leaf batches of 8 / 24 / 32 lose 1-2%).
+static constexpr int kRefillLanes = 24;
 static constexpr int kLeafBatch = 16;
 static constexpr int32_t kNoNode = -1; // never a leaf reference (those stay above 0xC0000000)
 
@@ -600,22 +604,32 @@ __global__ TPT_TRAVERSAL_ATTR void k_extend_pt(DScene S, DPaths P, const int32_t
     }
 }
 
-// Chunked traversal: wave w owns queue entries [w * chunk, (w + 1) * chunk) and
+// Entries per wave of a chunked launch over n rays: `chunk`, reduced in steps
+// of one wave's width so that the rays still spread over at least `waves`
+// wavefronts (late bounces and small renders have few rays).
+__device__ __forceinline__ uint32_t chunk_of(uint32_t n, int chunk, int waves) {
+    uint32_t c = (n + (uint32_t)waves - 1) / (uint32_t)waves;
+    c = (c + 63u) / 64u * 64u;
+    return max(64u, min((uint32_t)chunk, c));
+}
+
+// Chunked traversal: wave w owns queue entries [w * ck, (w + 1) * ck) and
 // refills its idle lanes from them (FetchLocal), so lanes stay busy while the
 // wave's rays stay neighbours in the queue; no shared counter.
 template <bool COUNT, int NL, bool POST, bool LDS>
 __global__ void k_extend_chunk(DScene S, DPaths P, const int32_t *queue, int qsel, int nsel_next, int chunk,
-                               int cache) {
+                               int waves, int cache) {
     stage_top_nodes(S, cache);
     uint32_t n = P.counters[qsel * kCntLine];
+    const uint32_t ck = chunk_of(n, chunk, waves);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.counters[nsel_next * kCntLine] = 0u; // next ray queue
         P.counters[kCntShadow] = 0u;
         if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 4), (unsigned long long)n);
     }
-    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)ck;
     if (begin >= n) return; // wave-uniform
-    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t end = (uint32_t)min(begin + (uint64_t)ck, (uint64_t)n);
     uint32_t nodes = 0, tris = 0;
     SrcQueue src{queue, P.org, P.dir, P.key, P.depth, P.hit};
     FetchLocal f{(uint32_t)begin};
@@ -627,14 +641,15 @@ __global__ void k_extend_chunk(DScene S, DPaths P, const int32_t *queue, int qse
 }
 
 template <bool COUNT, int NL, bool POST, bool LDS>
-__global__ void k_shadow_chunk(DScene S, DPaths P, int chunk, int cache) {
+__global__ void k_shadow_chunk(DScene S, DPaths P, int chunk, int waves, int cache) {
     stage_top_nodes(S, cache);
     uint32_t n = P.counters[kCntShadow];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long *>(P.counters + kCntTraffic + 6), (unsigned long long)n);
-    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    const uint32_t ck = chunk_of(n, chunk, waves);
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)ck;
     if (begin >= n) return;
-    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t end = (uint32_t)min(begin + (uint64_t)ck, (uint64_t)n);
     uint32_t nodes = 0, tris = 0;
     SrcShadow src{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
     FetchLocal f{(uint32_t)begin};
@@ -642,11 +657,13 @@ __global__ void k_shadow_chunk(DScene S, DPaths P, int chunk, int cache) {
 }
 
 template <bool ANY, bool COUNT, int NL, bool POST, bool LDS>
-__global__ void k_trace_chunk(DScene S, SrcRays src, uint32_t n, uint32_t *counters, int chunk, int cache) {
+__global__ void k_trace_chunk(DScene S, SrcRays src, uint32_t n, uint32_t *counters, int chunk, int waves,
+                              int cache) {
     stage_top_nodes(S, cache);
-    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)chunk;
+    const uint32_t ck = chunk_of(n, chunk, waves);
+    uint64_t begin = (uint64_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * (uint64_t)ck;
     if (begin >= n) return;
-    uint32_t end = (uint32_t)min(begin + (uint64_t)chunk, (uint64_t)n);
+    uint32_t end = (uint32_t)min(begin + (uint64_t)ck, (uint64_t)n);
     uint32_t nodes = 0, tris = 0;
     FetchLocal f{(uint32_t)begin};
     persistent_trace<ANY, COUNT, NL, POST, LDS>(S, src, end, f, nodes, tris, cache);
@@ -1434,55 +1451,73 @@ __global__ void k_shadow_batch_lane(DBatch B, DPaths P) {
                                    });
 }
 
-// Chunks of the scenes' segments for the chunked traversal kernels: out[s] =
-// exclusive prefix over scenes of ceil(count_s / chunk), out[n] = the total.
+// Chunks of the scenes' segments for the chunked traversal kernels: the chunk
+// size ck = chunk_of(total rays, chunk, waves) goes to out[n + 1]; out[s] =
+// exclusive prefix over scenes of ceil(count_s / ck), out[n] = the total.
 // One workgroup of 1024 threads, at most 1024 scenes.
-__global__ void k_batch_chunks(DBatch B, int fbase, int fcnt, int fout, int chunk) {
-    __shared__ int32_t buf[1024];
+// Inclusive scan over the workgroup; `total` gets the sum.  `buf` may be reused
+// as soon as it returns.
+__device__ __forceinline__ int block_scan_1024(int32_t *buf, int v, int &total) {
     int t = threadIdx.x;
-    int c = t < B.n ? (TPT_SEG(B, fcnt, t) + chunk - 1) / chunk : 0;
-    buf[t] = c;
+    buf[t] = v;
     __syncthreads();
     for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int v = t >= off ? buf[t - off] : 0;
+        int u = t >= off ? buf[t - off] : 0;
         __syncthreads();
-        buf[t] += v;
+        buf[t] += u;
         __syncthreads();
     }
-    if (t < B.n) TPT_SEG(B, fout, t) = buf[t] - c;
-    if (t == B.n - 1) TPT_SEG(B, fout, B.n) = buf[t];
+    int r = buf[t];
+    total = buf[blockDim.x - 1];
+    __syncthreads();
+    return r;
+}
+__global__ void k_batch_chunks(DBatch B, int fcnt, int fout, int chunk, int waves) {
+    __shared__ int32_t buf[1024];
+    int t = threadIdx.x;
+    int cnt = t < B.n ? TPT_SEG(B, fcnt, t) : 0;
+    int total = 0;
+    block_scan_1024(buf, cnt, total);
+    const int ck = (int)chunk_of((uint32_t)total, chunk, waves);
+    int c = (cnt + ck - 1) / ck;
+    int incl = block_scan_1024(buf, c, total);
+    if (t < B.n) TPT_SEG(B, fout, t) = incl - c;
+    if (t == B.n - 1) {
+        TPT_SEG(B, fout, B.n) = incl;
+        TPT_SEG(B, fout, B.n + 1) = ck;
+    }
 }
 
 // Chunked batch traversal: wave w takes chunk w of the concatenated per-scene
 // chunk lists, i.e. `chunk` consecutive entries of one scene's segment, and
 // refills its idle lanes from them; the scene is uniform across the wave.
 template <bool ANY, int NL, bool POST, bool LDS, class MakeSrc>
-__device__ __forceinline__ void batch_chunk_rays(const DBatch &B, int fbase, int fcnt, int fchunk, int chunk,
-                                                 MakeSrc make_src) {
+__device__ __forceinline__ void batch_chunk_rays(const DBatch &B, int fbase, int fcnt, int fchunk, MakeSrc make_src) {
     const int32_t *pre = B.seg + fchunk * B.seg_stride;
     int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     if (w >= pre[B.n]) return;
+    const uint32_t ck = (uint32_t)pre[B.n + 1];
     int s = __builtin_amdgcn_readfirstlane(find_seg(pre, B.n, w));
     uint32_t base = (uint32_t)TPT_SEG(B, fbase, s), cnt = (uint32_t)TPT_SEG(B, fcnt, s);
-    uint32_t begin = base + (uint32_t)(w - pre[s]) * (uint32_t)chunk;
-    uint32_t end = min(begin + (uint32_t)chunk, base + cnt);
+    uint32_t begin = base + (uint32_t)(w - pre[s]) * ck;
+    uint32_t end = min(begin + ck, base + cnt);
     uint32_t nodes = 0, tris = 0;
     FetchLocal f{begin};
     persistent_trace<ANY, false, NL, POST, LDS>(scene_at(B, s), make_src(), end, f, nodes, tris, 0);
 }
 
 template <int NL, bool POST, bool LDS>
-__global__ void k_extend_batch_chunk(DBatch B, DPaths P, int qsel, int chunk) {
+__global__ void k_extend_batch_chunk(DBatch B, DPaths P, int qsel) {
     const int fb = qsel ? kSegQbBase : kSegQaBase;
     const int32_t *queue = qsel ? P.queue_b : P.queue_a;
-    batch_chunk_rays<false, NL, POST, LDS>(B, fb, fb + 1, kSegRayChunk, chunk, [&]() {
+    batch_chunk_rays<false, NL, POST, LDS>(B, fb, fb + 1, kSegRayChunk, [&]() {
         return SrcQueue{queue, P.org, P.dir, P.key, P.depth, P.hit};
     });
 }
 
 template <int NL, bool POST, bool LDS>
-__global__ void k_shadow_batch_chunk(DBatch B, DPaths P, int chunk) {
-    batch_chunk_rays<true, NL, POST, LDS>(B, kSegShBase, kSegShCnt, kSegShChunk, chunk, [&]() {
+__global__ void k_shadow_batch_chunk(DBatch B, DPaths P) {
+    batch_chunk_rays<true, NL, POST, LDS>(B, kSegShBase, kSegShCnt, kSegShChunk, [&]() {
         return SrcShadow{P.shadow_org, P.shadow_dir, P.shadow_val, P.shadow_slot, P.color};
     });
 }
@@ -1590,17 +1625,13 @@ static size_t trav_shared(const DScene &S, int nl, bool lds_stack) {
     return nodes + (lds_stack ? lds_bytes(S.trav_block) : 0);
 }
 
-// Chunk size of a launch over at most `max_rays` entries: the configured size,
-// reduced (down to one wave's worth) so that the launch still has 16 waves per
-// CU when the rays allow it; small renders would otherwise leave CUs idle.
-static int chunk_for(uint64_t max_rays, int chunk, int num_cu) {
-    uint64_t c = max_rays / ((uint64_t)num_cu * 16) / 64 * 64;
-    return (int)std::max<uint64_t>(64, std::min<uint64_t>((uint64_t)chunk, c));
-}
-
-// Workgroups covering `max_rays` queue entries at `chunk` entries per wave.
-static int chunk_grid(uint64_t max_rays, int chunk, int block) {
-    uint64_t waves = (max_rays + (uint64_t)chunk - 1) / (uint64_t)chunk;
+// Chunked launches spread their rays over at least 16 wavefronts per CU
+// (chunk_of on the device); the grid covers max(max_rays / chunk, that many)
+// wavefronts, plus `extra` (one partial chunk per scene of a batch).
+static int chunk_waves(int num_cu) { return num_cu * 16; }
+static int chunk_grid(uint64_t max_rays, int chunk, int block, int num_cu, uint64_t extra = 0) {
+    uint64_t waves = std::max<uint64_t>((max_rays + (uint64_t)chunk - 1) / (uint64_t)chunk, (uint64_t)chunk_waves(num_cu));
+    waves += extra;
     uint64_t per_block = (uint64_t)block / 64;
     return (int)std::max<uint64_t>(1, (waves + per_block - 1) / per_block);
 }
@@ -1621,12 +1652,12 @@ void launch_extend(const DScene &S, const DPaths &P, const int32_t *queue, int q
             }
             if (S.persistent >= 3) {
                 LaneLds m = lane_lds(S, NL, block);
-                int ck = chunk_for(max_rays, S.chunk, S.num_cu);
-                int g = chunk_grid(max_rays, ck, block);
+                int g = chunk_grid(max_rays, S.chunk, block, S.num_cu);
                 by_bool(S.persistent == 4, [&](auto Q) {
                     by_bool(m.stack, [&](auto D) {
                         hipLaunchKernelGGL((k_extend_chunk<CNT, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
-                                           dim3(block), m.bytes, st, S, P, queue, qsel, nsel, ck, m.cache);
+                                           dim3(block), m.bytes, st, S, P, queue, qsel, nsel, S.chunk,
+                                           chunk_waves(S.num_cu), m.cache);
                     });
                 });
                 return;
@@ -1665,12 +1696,11 @@ void launch_shadow(const DScene &S, const DPaths &P, int grid, int block, bool c
             }
             if (S.persistent >= 3) {
                 LaneLds m = lane_lds(S, NL, block);
-                int ck = chunk_for(max_rays, S.chunk, S.num_cu);
-                int g = chunk_grid(max_rays, ck, block);
+                int g = chunk_grid(max_rays, S.chunk, block, S.num_cu);
                 by_bool(S.persistent == 4, [&](auto Q) {
                     by_bool(m.stack, [&](auto D) {
                         hipLaunchKernelGGL((k_shadow_chunk<CNT, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
-                                           dim3(block), m.bytes, st, S, P, ck, m.cache);
+                                           dim3(block), m.bytes, st, S, P, S.chunk, chunk_waves(S.num_cu), m.cache);
                     });
                 });
                 return;
@@ -1710,13 +1740,12 @@ void launch_trace_closest(const DScene &S, const float4 *rays, uint64_t n, int32
             SrcRays src{rays, prim, t, nullptr};
             if (S.persistent >= 3) {
                 LaneLds m = lane_lds(S, NL, block);
-                int ck = chunk_for(n, S.chunk, S.num_cu);
-                int g = chunk_grid(n, ck, block);
+                int g = chunk_grid(n, S.chunk, block, S.num_cu);
                 by_bool(S.persistent == 4, [&](auto Q) {
                     by_bool(m.stack, [&](auto D) {
                         hipLaunchKernelGGL((k_trace_chunk<false, CNT, NL, decltype(Q)::value, decltype(D)::value>),
-                                           dim3(g), dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, ck,
-                                           m.cache);
+                                           dim3(g), dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, S.chunk,
+                                           chunk_waves(S.num_cu), m.cache);
                     });
                 });
                 return;
@@ -1747,12 +1776,12 @@ void launch_trace_occluded(const DScene &S, const float4 *rays, uint64_t n, uint
         SrcRays src{rays, nullptr, nullptr, occ};
         if (S.persistent >= 3) {
             LaneLds m = lane_lds(S, NL, block);
-            int ck = chunk_for(n, S.chunk, S.num_cu);
-            int g = chunk_grid(n, ck, block);
+            int g = chunk_grid(n, S.chunk, block, S.num_cu);
             by_bool(S.persistent == 4, [&](auto Q) {
                 by_bool(m.stack, [&](auto D) {
                     hipLaunchKernelGGL((k_trace_chunk<true, false, NL, decltype(Q)::value, decltype(D)::value>), dim3(g),
-                                       dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, ck, m.cache);
+                                       dim3(block), m.bytes, st, S, src, (uint32_t)n, counters, S.chunk,
+                                       chunk_waves(S.num_cu), m.cache);
                 });
             });
             return;
@@ -1787,10 +1816,9 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
     if (mode >= 3) {
         // chunked traversal with the pipeline's block size: at most cap / chunk + n chunks
-        chunk = chunk_for((uint64_t)B.cap, chunk, num_cu);
         bool stk = lds_stack && lds_bytes(block) <= (size_t)max_lds;
         size_t sh = stk ? lds_bytes(block) : 0;
-        int cg = chunk_grid((uint64_t)B.cap + (uint64_t)B.n * (uint64_t)chunk, chunk, block);
+        int cg = chunk_grid((uint64_t)B.cap, chunk, block, num_cu, (uint64_t)B.n);
         by_layout(layout, [&](auto L) {
             by_bool(mode == 4, [&](auto Q) {
                 by_bool(stk, [&](auto K) {
@@ -1799,16 +1827,16 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
                     for (int b = 0; b <= max_depth; b++) {
                         int qsel = b & 1;
                         int fb = qsel ? kSegQbBase : kSegQaBase;
-                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, fb, fb + 1, (int)kSegRayChunk,
-                                           chunk);
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, fb + 1, (int)kSegRayChunk,
+                                           chunk, chunk_waves(num_cu));
                         hipLaunchKernelGGL((k_extend_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B, P,
-                                           qsel, chunk);
+                                           qsel);
                         hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
                         hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
-                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, (int)kSegShBase,
-                                           (int)kSegShCnt, (int)kSegShChunk, chunk);
-                        hipLaunchKernelGGL((k_shadow_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B, P,
-                                           chunk);
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, (int)kSegShCnt,
+                                           (int)kSegShChunk, chunk, chunk_waves(num_cu));
+                        hipLaunchKernelGGL((k_shadow_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B,
+                                           P);
                     }
                 });
             });
