@@ -106,6 +106,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache", "city"], default="room")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache: samples per pixel of the reference")
+    ap.add_argument("--cache-suffix", type=int, default=3,
+                    help="cache: training paths end in the cache this many vertices past their cache vertex "
+                         "(0: they run to the maximum depth)")
+    ap.add_argument("--cache-frames", type=str, default="16,32,64,128,256", help="cache: frame counts of the rows")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--no-seq", action="store_true", help="batch: skip the one-by-one comparison")
     ap.add_argument("--batch-mix", choices=["mixed", "small"], default="mixed",
@@ -336,11 +340,11 @@ def bench_cache(args):
         print(f"# plain {spp:4d} spp: {dt * 1e3:8.1f} ms relMSE {plain[-1][2]:.5f}", file=sys.stderr, flush=True)
     c = sum(e * dt for _, dt, e in plain) / len(plain)  # relMSE * time of the plain tracer
     cached = []
-    tinypt_amd.render_cached(dev, 2, cache_depth=2, reset=True, seed=3, **kw)  # warm-up
-    for frames in (16, 32, 64, 128, 256):
+    tinypt_amd.render_cached(dev, 2, cache_depth=2, reset=True, seed=3, suffix_len=args.cache_suffix, **kw)  # warm-up
+    for frames in [int(x) for x in args.cache_frames.split(",")]:
         t = time.perf_counter()
         img, st = tinypt_amd.render_cached(dev, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3,
-                                           reset=True, weight_seed=5, seed=1, **kw)
+                                           reset=True, weight_seed=5, seed=1, suffix_len=args.cache_suffix, **kw)
         dt = time.perf_counter() - t
         e = rel(img)
         cached.append(dict(frames=frames, ms=round(dt * 1e3, 1), relmse=round(e, 5),
@@ -353,7 +357,8 @@ def bench_cache(args):
     paths = W * H * best["frames"]
     out = base_line(paths / (best["ms"] / 1e3) / 1e6, best["ms"], args, {
         "workload": f"textured_room_264612tri_{W}x{H}_depth12_rr5_rgba8_alpha_bump",
-        "tracer": "radiance cache (5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, cache after 2 bounces)",
+        "tracer": (f"radiance cache (5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, cache after 2 bounces, "
+                   f"training suffixes end in the cache after {args.cache_suffix} more vertices)"),
         "paths_per_step": paths,
         "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation",
     })

@@ -252,6 +252,10 @@ typedef struct {
     float learning_rate;   /* Adam step size */
     int32_t train_steps;   /* Adam steps per frame, each on 1 / train_steps of the frame's records (>= 1) */
     uint64_t weight_seed;
+    int32_t suffix_len;    /* > 0: a training path also terminates into the cache, suffix_len vertices
+                              after its cache vertex (self-training: its target then includes the cache's
+                              prediction there); 0: training paths run to max_depth */
+    int32_t pad_;
 } tpt_cache_params;
 typedef struct {
     double render_ms;      /* wall time of the render */

@@ -118,7 +118,10 @@ def test_progressive_without_cache_equals_plain_render(hip, dev):
     assert st["queries"] == 0 and st["records"] == 0
 
 
-def test_cached_render_trains_online(hip):
+@pytest.mark.parametrize("suffix_len", [0, 3])
+def test_cached_render_trains_online(hip, suffix_len):
+    """suffix_len 0: training paths run to max_depth; 3: they end in the cache
+    three vertices past their cache vertex (self-training)."""
     from tinypt_amd import scenes
 
     dev = scenes.triangle_room(128, 96).to("hip")
@@ -127,7 +130,7 @@ def test_cached_render_trains_online(hip):
     plain, _ = hip.render_cached(dev, 64, cache_depth=-1, seed=1, **kw)
     # 12,288 pixels, 2048 training records per frame (about one path in six), 4 Adam steps per frame
     img, st = hip.render_cached(dev, 64, cache_depth=2, train_records=2048, learning_rate=1e-3, reset=True,
-                                weight_seed=7, seed=1, **kw)
+                                weight_seed=7, seed=1, suffix_len=suffix_len, **kw)
     print(st)
     assert np.all(np.isfinite(img)) and img.min() >= 0.0
     assert st["queries"] > 5000 and 0 < st["records"] <= 2048
@@ -141,5 +144,5 @@ def test_cached_render_trains_online(hip):
     assert abs(img.mean() - ref.mean()) < 0.05 * ref.mean()  # no visible bias
     assert rel(img) < rel(plain)  # lower error than the plain estimator at the same frames
     # the network keeps training across calls
-    _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, **kw)
+    _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, suffix_len=suffix_len, **kw)
     assert st2["steps"] == 64 * 4 + 2 * 4

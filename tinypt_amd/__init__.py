@@ -97,6 +97,8 @@ class _CacheParams(ctypes.Structure):
         ("learning_rate", ctypes.c_float),
         ("train_steps", ctypes.c_int32),
         ("weight_seed", ctypes.c_uint64),
+        ("suffix_len", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 
@@ -214,10 +216,13 @@ def scene_info(scene_hip):
 
 # ---------------------------------------------------------------- learned radiance cache
 def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3, reset=False,
-                  weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=4):
+                  weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=4,
+                  suffix_len=3):
     """Progressive render with the learned radiance cache (cache_depth < 0: plain
-    progressive frames).  Returns (H x W x 3 float32 HDR mean, row 0 at the
-    bottom, stats dict)."""
+    progressive frames).  suffix_len > 0 ends training paths in the cache too,
+    suffix_len vertices past their cache vertex (self-training); 0 traces them
+    to max_depth.  Returns (H x W x 3 float32 HDR
+    mean, row 0 at the bottom, stats dict)."""
     import numpy as np
 
     cam = scene_hip.camera()
@@ -225,7 +230,8 @@ def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learn
     p = _RenderParams(spp=1, max_depth=max_depth, rr_depth=rr_depth, shadow_bounded=int(shadow_bounded), seed=seed,
                       output=1, accum_fp64=int(fp64))
     c = _CacheParams(frames=frames, cache_depth=cache_depth, train_records=train_records, reset=int(reset),
-                     learning_rate=learning_rate, train_steps=train_steps, weight_seed=weight_seed)
+                     learning_rate=learning_rate, train_steps=train_steps, weight_seed=weight_seed,
+                     suffix_len=suffix_len)
     st = _CacheStats()
     _check(lib().tpt_hip_render_cached(scene_hip._hip_ptr(), ctypes.byref(p), ctypes.byref(c), out.ctypes.data,
                                        ctypes.byref(st)))

@@ -96,7 +96,7 @@ struct CacheState {
     void *bmem = nullptr; // per-slot buffers for `slots` paths
     size_t slots = 0;
     int train_cap = 0;
-    uint16_t *feat = nullptr;
+    uint16_t *feat = nullptr, *feat_t = nullptr;
     float4 *rec_T = nullptr, *rec_C = nullptr, *pred = nullptr, *target = nullptr;
     int32_t *q_slot = nullptr, *t_slot = nullptr;
     ~CacheState() {
@@ -859,11 +859,13 @@ void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
         off = align256(off + bytes);
         return o;
     };
-    size_t o_f = take(slots * kNrcInputs * 2), o_T = take(slots * 16), o_C = take(slots * 16), o_p = take(slots * 16),
+    size_t o_f = take(slots * kNrcInputs * 2), o_ft = take(slots * kNrcInputs * 2), o_T = take(slots * 16),
+           o_C = take(slots * 16), o_p = take(slots * 16),
            o_q = take(slots * 4), o_t = take((size_t)train_cap * 4), o_tg = take((size_t)train_cap * 16);
     HIP_OK(hipMalloc(&c.bmem, off));
     uint8_t *b = (uint8_t *)c.bmem;
     c.feat = (uint16_t *)(b + o_f);
+    c.feat_t = (uint16_t *)(b + o_ft);
     c.rec_T = (float4 *)(b + o_T);
     c.rec_C = (float4 *)(b + o_C);
     c.pred = (float4 *)(b + o_p);
@@ -886,6 +888,7 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
     if (!(cp->learning_rate >= 0.f)) return set_err(TPT_ERR_INVALID_ARGUMENT, "learning_rate must not be negative");
     if (cp->train_steps < 1 || cp->train_steps > cp->train_records)
         return set_err(TPT_ERR_INVALID_ARGUMENT, "train_steps must be in 1 .. train_records");
+    if (cp->suffix_len < 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "suffix_len must not be negative");
     try {
         auto t0 = std::chrono::steady_clock::now();
         HIP_OK(hipSetDevice(h->device));
@@ -916,7 +919,9 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
             C.lo[a] = h->bounds_lo[a];
             C.inv_ext[a] = 1.f / (h->bounds_hi[a] - h->bounds_lo[a]);
         }
+        C.suffix = cp->suffix_len;
         C.feat = c.feat;
+        C.feat_t = c.feat_t;
         C.rec_T = c.rec_T;
         C.rec_C = c.rec_C;
         C.q_slot = c.q_slot;
@@ -956,7 +961,7 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
                 // train_steps Adam steps on consecutive mini-batches of the frame's records
                 const int K = cp->train_steps, mb = (cp->train_records + K - 1) / K;
                 for (int k = 0; k < K; k++) {
-                    nrc_launch_train(c.feat, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
+                    nrc_launch_train(c.feat_t, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
                     nrc_launch_adam(c.W, cp->learning_rate, ++c.step, st);
                 }
                 HIP_OK(hipEventRecord(e[3], st));

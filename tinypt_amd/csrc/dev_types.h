@@ -223,7 +223,10 @@ struct DCache {
     int32_t train_cap;    // capacity of the training-record list
     float lo[3];          // scene bounds: position encoding (p - lo) * inv_ext
     float inv_ext[3];
-    uint16_t *feat;       // per slot: 64 BF16 inputs of its cache vertex
+    int32_t suffix;       // > 0: training paths terminate into the cache `suffix` vertices after `depth`
+    int32_t pad_;
+    uint16_t *feat;       // per slot: 64 BF16 inputs of the vertex where the path queries the cache
+    uint16_t *feat_t;     // per slot: 64 BF16 inputs of a training path's cache vertex
     float4 *rec_T;        // per slot: throughput arriving at the vertex (training paths)
     float4 *rec_C;        // per slot: radiance after the vertex's emission (training paths)
     int32_t *q_slot;      // query list (slots), count in counters[0]

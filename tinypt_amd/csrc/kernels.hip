@@ -893,15 +893,21 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
         const DMaterial &m = S.materials[mi];
         color = add(color, cmul(decay, color_at(S, m.emission, sh.uv)));
         bool into_cache = false;
-        if (C && depth == C->depth) {
+        // the cache vertex, and the end of a training path's suffix (only training
+        // paths get that far: the others end at the cache vertex)
+        const bool at_cache = C && depth == C->depth;
+        const bool suffix_end = C && C->suffix > 0 && depth == C->depth + C->suffix;
+        if (at_cache || suffix_end) {
             V3 hp = at(o, d, h.t);
             V3 p01{(hp.x - C->lo[0]) * C->inv_ext[0], (hp.y - C->lo[1]) * C->inv_ext[1],
                    (hp.z - C->lo[2]) * C->inv_ext[2]};
             float rough;
             V3 alb;
             lobe_summary(S, m, sh.uv, rough, alb);
-            nrc_encode(C->feat + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal, rough, alb);
-            if (cache_trains(*C, slot)) {
+            const bool trains = at_cache && cache_trains(*C, slot);
+            nrc_encode((trains ? C->feat_t : C->feat) + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal, rough,
+                       alb);
+            if (trains) {
                 out_.cache_kind = 2;
                 C->rec_T[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
                 C->rec_C[slot] = make_float4(color.x, color.y, color.z, 0.f);
