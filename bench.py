@@ -44,7 +44,7 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
 # FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
 # read inside this process, so the figure is the profiled one; it is reported only for that workload.
-EXTEND_TRAFFIC_KIB = {"fetch": 1235349.8, "fetch_scale": 2.0, "write": 283019.1, "write_scale": 1.0}
+EXTEND_TRAFFIC_KIB = {"fetch": 1230183.4, "fetch_scale": 2.0, "write": 272680.2, "write_scale": 1.0}
 DEFAULT_ROOM = (1024, 768, 256)
 
 
