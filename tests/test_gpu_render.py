@@ -123,3 +123,18 @@ def test_render_is_deterministic(hip):
     a = pt.render(dev, 8, seed=4).raw()
     b = pt.render(dev, 8, seed=4, queue_capacity=64 * 48 * 2, block_size=64).raw()
     assert np.array_equal(a, b)
+
+
+def test_traversal_kernels_render_identically(hip, monkeypatch):
+    # every traversal kernel family (TPT_PERSISTENT 0-4) finds the same closest hits,
+    # so the same seed renders the same image bit for bit
+    from tinypt_amd import scenes
+
+    host = scenes.triangle_room(96, 64)
+    images = []
+    for mode in "01234":
+        monkeypatch.setenv("TPT_PERSISTENT", mode)
+        dev = host.to("hip")
+        images.append(hip.PathTracer("hip").render(dev, 16, seed=3, max_depth=8, rr_depth=5, hdr=True).raw())
+    for img in images[1:]:
+        assert np.array_equal(images[0], img)

@@ -115,6 +115,27 @@ def test_other_node_layouts_match_oracle(hip, monkeypatch, layout, node_bytes):
     assert np.array_equal(gt[gp == op], ot[gp == op])
 
 
+@pytest.mark.parametrize("mode", ["0", "1", "2", "3", "4"])
+def test_traversal_kernels_match_oracle(hip, monkeypatch, mode):
+    # TPT_PERSISTENT selects the traversal kernels: 0 one ray per lane, 1 / 2 persistent
+    # with a launch-wide refill counter (2: parked leaves), 3 / 4 per-wave chunks (4, the
+    # default: parked leaves).  Closest hits do not depend on the visiting order.
+    from tinypt_amd import scenes
+
+    monkeypatch.setenv("TPT_PERSISTENT", mode)
+    host = scenes.triangle_room(64, 48)
+    dev = host.to("hip")
+    oracle = O.OracleScene(host)
+    rays = raysets.extend_set(oracle, _room_cam(), 100_000, seed=23)
+    gp, gt = hip.trace_closest(dev, rays)
+    op, ot = oracle.trace_closest(rays, mode=0)
+    mism, ties, prune, bad = _classify(oracle, rays, gp, gt, op, ot)
+    assert not bad and len(mism) <= 20
+    assert np.array_equal(gt[gp == op], ot[gp == op])
+    occ = hip.trace_occluded(dev, rays)
+    assert np.array_equal(occ, (gp >= 0).astype(occ.dtype))  # unbounded segments: occluded iff hit
+
+
 def test_spheres_closest_hit(hip):
     from tinypt_amd import scenes
 
