@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define TPT_ABI_VERSION 1
+#define TPT_ABI_VERSION 2
 
 /* ---- error codes ------------------------------------------------------- */
 enum {

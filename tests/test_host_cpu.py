@@ -27,7 +27,7 @@ def test_c_abi_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(T.C_ABI_SYMBOLS)
-    assert T.lib().tpt_hip_abi_version() == 1
+    assert T.lib().tpt_hip_abi_version() == T.ABI_VERSION == 2
 
 
 def test_device_strings():

@@ -135,6 +135,9 @@ C_ABI_SYMBOLS = (
 )
 
 
+ABI_VERSION = 2  # TPT_ABI_VERSION of include/tinypt_hip.h (the ctypes structures above follow it)
+
+
 def lib():
     """ctypes handle on libtinypt_hip.so with the C ABI prototypes set."""
     global _lib
@@ -161,6 +164,9 @@ def lib():
     for name in C_ABI_SYMBOLS:
         if name not in ("tpt_hip_abi_version", "tpt_hip_device_count", "tpt_hip_last_error", "tpt_hip_scene_destroy"):
             getattr(L, name).restype = ctypes.c_int
+    if L.tpt_hip_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI version {L.tpt_hip_abi_version()}, this package expects "
+                           f"{ABI_VERSION}: rebuild the library (make -C tinypt_amd/csrc)")
     _lib = L
     return L
 
