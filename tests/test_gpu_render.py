@@ -138,3 +138,68 @@ def test_traversal_kernels_render_identically(hip, monkeypatch):
         images.append(hip.PathTracer("hip").render(dev, 16, seed=3, max_depth=8, rr_depth=5, hdr=True).raw())
     for img in images[1:]:
         assert np.array_equal(images[0], img)
+
+
+def test_obj_scene_matches_oracle(hip, tmp_path):
+    # a room loaded through TriangleMesh.from_obj (y-up OBJ, quads, an MTL with a
+    # textured floor, a glossy wall, metal and glass boxes) renders like the oracle
+    import math
+
+    from PIL import Image as PILImage
+
+    from tinypt_amd import scenes
+
+    quads, lines = [], []
+    verts = []
+
+    def quad(p0, p1, p2, p3, mat):
+        base = len(verts)
+        verts.extend([p0, p1, p2, p3])
+        quads.append((mat, [base + 1, base + 2, base + 3, base + 4]))
+
+    S = 10.0
+    quad((0, 0, 0), (S, 0, 0), (S, 0, S), (0, 0, S), "floor")  # y = 0 (becomes z = 0)
+    quad((0, S, 0), (0, S, S), (S, S, S), (S, S, 0), "white")  # ceiling
+    quad((0, 0, 0), (0, 0, S), (0, S, S), (0, S, 0), "red")
+    quad((S, 0, 0), (S, S, 0), (S, S, S), (S, 0, S), "glossy")
+    quad((0, 0, S), (S, 0, S), (S, S, S), (0, S, S), "white")
+    for (x0, z0, m) in ((2, 2, "metal"), (6, 5, "glass")):  # two 2x2x2 boxes
+        x1, z1, h = x0 + 2, z0 + 2, 2.0
+        quad((x0, h, z0), (x0, h, z1), (x1, h, z1), (x1, h, z0), m)
+        quad((x0, 0, z0), (x1, 0, z0), (x1, h, z0), (x0, h, z0), m)
+        quad((x0, 0, z1), (x0, h, z1), (x1, h, z1), (x1, 0, z1), m)
+        quad((x0, 0, z0), (x0, h, z0), (x0, h, z1), (x0, 0, z1), m)
+        quad((x1, 0, z0), (x1, 0, z1), (x1, h, z1), (x1, h, z0), m)
+    lines.append("mtllib room.mtl")
+    lines += [f"v {x} {y} {z}" for x, y, z in verts]
+    lines += ["vt 0 0", "vt 4 0", "vt 4 4", "vt 0 4"]
+    for mat, idx in quads:
+        lines.append(f"usemtl {mat}")
+        lines.append("f " + " ".join(f"{i}/{k + 1}" for k, i in enumerate(idx)))
+    (tmp_path / "room.obj").write_text("\n".join(lines) + "\n")
+    (tmp_path / "room.mtl").write_text(
+        "newmtl floor\nKd 1 1 1\nmap_Kd floor.png\n"
+        "newmtl white\nKd 0.75 0.75 0.75\n"
+        "newmtl red\nKd 0.75 0.25 0.25\n"
+        "newmtl glossy\nKd 0.3 0.3 0.5\nKs 0.4 0.4 0.4\nNs 80\n"
+        "newmtl metal\nillum 3\nKs 0.9 0.85 0.8\n"
+        "newmtl glass\nillum 7\nNi 1.5\n")
+    y, x = np.mgrid[0:32, 0:32]
+    checker = ((x // 8 + y // 8) % 2)[..., None] * np.array([200, 120, 40]) + 40
+    PILImage.fromarray(checker.astype(np.uint8), "RGB").save(tmp_path / "floor.png")
+
+    mesh = hip.TriangleMesh.from_obj(str(tmp_path / "room.obj"))
+    assert mesh.num_faces == 2 * len(quads)
+    light = hip.Rectangle()
+    light.location = [5.0, -5.0, 9.9]  # OBJ (x, y, z) is (x, -z, y) here: the room spans y in [-10, 0]
+    light.dimension = [3.0, 3.0]
+    light.material = scenes.material(hip.Lambertian(scenes.rgb((0, 0, 0))), emission=(12, 12, 12))
+    cam = hip.Camera()
+    cam.location = [5.0, -0.5, 5.0]  # in the open side of the room, facing the back wall (OBJ z = 10)
+    cam.rotation = scenes.lookat_rotation((0.0, -1.0, -0.1))
+    cam.width, cam.height = 80, 60
+    cam.fov = math.radians(60.0)
+    host = hip.Scene(cam, [mesh, light], [], None)
+    g, o, o2 = render_both(hip, host, 16, seed=21, hdr=True, max_depth=6, rr_depth=4)
+    assert g.mean() > 0.01  # the light reaches the camera
+    check(g, o, o2, 0.9)

@@ -83,6 +83,12 @@ py::dict stats_dict(const tpt_render_stats &s) {
     return d;
 }
 
+py::array_t<float> rows_f32(const std::vector<float> &v, int cols) {
+    py::array_t<float> a({(py::ssize_t)(v.size() / cols), (py::ssize_t)cols});
+    if (!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * 4);
+    return a;
+}
+
 std::shared_ptr<TriangleMesh> mesh_from_arrays(py::array_t<float, py::array::c_style | py::array::forcecast> vertices,
                                                py::array_t<int32_t, py::array::c_style | py::array::forcecast> faces,
                                                std::vector<Material> materials, py::object face_material,
@@ -204,6 +210,12 @@ PYBIND11_MODULE(_C, m) {
         .def_property_readonly("height", &Image::height)
         .def_property_readonly("channels", &Image::channels)
         .def("raw", &image_raw)
+        .def_static("open", [](const std::string &path, bool grayscale) {
+            RGBAImage r = Image::open(path, grayscale);
+            return py::make_tuple(r.rgb ? py::cast(r.rgb) : py::none(), r.alpha ? py::cast(r.alpha) : py::none());
+        }, py::arg("path"), py::arg("grayscale") = false,
+           "(rgb, alpha) of an image file, as cpu::Image::open reads it: linear RGB, row 0 at the bottom")
+        .def("save", &Image::save, py::arg("path"))
         .def("numpy", [](const Image &self) {
             py::array_t<float> a({self.height(), self.width(), self.channels()});
             float *d = a.mutable_data();
@@ -270,6 +282,7 @@ PYBIND11_MODULE(_C, m) {
         .def_readwrite("material", &Circle::material);
     py::class_<TriangleMesh, Object, std::shared_ptr<TriangleMesh>>(m, "TriangleMesh")
         .def(py::init())
+        .def_static("from_obj", &TriangleMesh::from_obj, py::arg("path"), py::arg("material") = nullptr)
         .def_static("from_arrays", &mesh_from_arrays, py::arg("vertices"), py::arg("faces"),
                     py::arg("materials") = std::vector<Material>{}, py::arg("face_material") = py::none(),
                     py::arg("normals") = py::none(), py::arg("texcoords") = py::none(),
@@ -277,7 +290,19 @@ PYBIND11_MODULE(_C, m) {
         .def_readwrite("location", &TriangleMesh::location)
         .def_property("rotation", [](const TriangleMesh &o) { return from_mat3(o.rotation); },
                       [](TriangleMesh &o, py::array_t<float> r) { o.rotation = to_mat3(r); })
-        .def_property_readonly("num_faces", &TriangleMesh::num_faces);
+        .def_property_readonly("num_faces", &TriangleMesh::num_faces)
+        // read-only views of the mesh arrays (faces: F x 9 = (vertex, normal, texcoord) per corner, -1 when absent)
+        .def_property_readonly("vertices", [](const TriangleMesh &t) { return rows_f32(t.vertices, 3); })
+        .def_property_readonly("normals", [](const TriangleMesh &t) { return rows_f32(t.normals, 3); })
+        .def_property_readonly("texcoords", [](const TriangleMesh &t) { return rows_f32(t.texcoords, 2); })
+        .def_property_readonly("faces", [](const TriangleMesh &t) {
+            py::array_t<int32_t> a({(py::ssize_t)t.num_faces(), (py::ssize_t)9});
+            std::memcpy(a.mutable_data(), t.faces.data(), t.faces.size() * 4);
+            return a;
+        })
+        .def_property_readonly("face_material", [](const TriangleMesh &t) { return py::array_t<int32_t>(t.face_material.size(), t.face_material.data()); })
+        .def_property_readonly("shape_offsets", [](const TriangleMesh &t) { return py::array_t<int32_t>(t.shape_offsets.size(), t.shape_offsets.data()); })
+        .def_property_readonly("materials", [](const TriangleMesh &t) { return t.materials; });
 
     // lights (pybind.cpp:138-152)
     py::class_<Light, std::shared_ptr<Light>>(m, "Light");
@@ -297,18 +322,29 @@ PYBIND11_MODULE(_C, m) {
 
     // bxdf (pybind.cpp:155-158)
     py::class_<BXDF, std::shared_ptr<BXDF>>(m, "BXDF");
-    py::class_<Lambertian, BXDF, std::shared_ptr<Lambertian>>(m, "Lambertian").def(py::init<RGBTexture>());
-    py::class_<Metal, BXDF, std::shared_ptr<Metal>>(m, "Metal").def(py::init<RGBTexture>());
-    py::class_<Dielectric, BXDF, std::shared_ptr<Dielectric>>(m, "Dielectric").def(py::init<RGBTexture, float>());
+    py::class_<Lambertian, BXDF, std::shared_ptr<Lambertian>>(m, "Lambertian")
+        .def(py::init<RGBTexture>())
+        .def_readonly("texture", &Lambertian::texture);
+    py::class_<Metal, BXDF, std::shared_ptr<Metal>>(m, "Metal")
+        .def(py::init<RGBTexture>())
+        .def_readonly("texture", &Metal::texture);
+    py::class_<Dielectric, BXDF, std::shared_ptr<Dielectric>>(m, "Dielectric")
+        .def(py::init<RGBTexture, float>())
+        .def_readonly("texture", &Dielectric::texture)
+        .def_readonly("ior", &Dielectric::ior);
     py::class_<Glossy, BXDF, std::shared_ptr<Glossy>> glossy(m, "Glossy");
     py::enum_<Glossy::MicrofacetModelType>(glossy, "MicrofacetModelType")
         .value("MICRO_BLINN_PHONG", Glossy::MICRO_BLINN_PHONG)
         .value("MICRO_TROWBRIDGE_REITZ", Glossy::MICRO_TROWBRIDGE_REITZ)
         .export_values();
     glossy.def(py::init<float, RGBTexture, Glossy::MicrofacetModelType>(), py::arg("shininess"), py::arg("texture"),
-               py::arg("micro_type") = Glossy::MICRO_BLINN_PHONG);
+               py::arg("micro_type") = Glossy::MICRO_BLINN_PHONG)
+        .def_readonly("shininess", &Glossy::shininess)
+        .def_readonly("texture", &Glossy::texture);
     py::class_<AddBXDF, BXDF, std::shared_ptr<AddBXDF>>(m, "AddBXDF")
-        .def(py::init<std::shared_ptr<BXDF>, std::shared_ptr<BXDF>>());
+        .def(py::init<std::shared_ptr<BXDF>, std::shared_ptr<BXDF>>())
+        .def_readonly("bxdf1", &AddBXDF::bxdf1)
+        .def_readonly("bxdf2", &AddBXDF::bxdf2);
     py::class_<MixBXDF, BXDF, std::shared_ptr<MixBXDF>>(m, "MixBXDF")
         .def(py::init<std::shared_ptr<BXDF>, std::shared_ptr<BXDF>, float>(), py::arg("bxdf1"), py::arg("bxdf2"),
              py::arg("weight") = 0.5f);
@@ -316,6 +352,7 @@ PYBIND11_MODULE(_C, m) {
     // material (pybind.cpp:161-166)
     py::class_<Material, std::shared_ptr<Material>>(m, "Material")
         .def(py::init<std::shared_ptr<BXDF>>())
+        .def_property_readonly("surface", &Material::surface)
         .def_property("emission_texture", &Material::emission_texture, &Material::set_emission_texture)
         .def_property("alpha_texture", &Material::alpha_texture, &Material::set_alpha_texture)
         .def_property("bump_texture", &Material::bump_texture, &Material::set_bump_texture);

@@ -78,12 +78,17 @@ class Device {
 
 // ---------------------------------------------------------------- Image (cpu/image.h:13-94)
 // Row-major, row 0 at the bottom (y up), like cpu::Image.  F32 or U8 texels.
+struct RGBAImage;
 class Image {
   public:
     Image() = default;
     Image(int width, int height, int channels) : _w(width), _h(height), _c(channels), _f((size_t)width * height * channels, 0.f) {}
     static Image from_f32(int w, int h, int c, const float *data);
     static Image from_u8(int w, int h, int c, const uint8_t *data);
+    // Image files (image_io.cpp), with the conventions of cpu::Image::open / save
+    // (cpu/image.cpp:6-72): linear RGB (pow 2.2) and alpha, row 0 at the bottom.
+    static RGBAImage open(const std::string &path, bool grayscale = false);
+    void save(const std::string &path) const;
     bool empty() const { return _w == 0 || _h == 0; }
     int width() const { return _w; }
     int height() const { return _h; }
@@ -101,6 +106,11 @@ class Image {
     int _w = 0, _h = 0, _c = 0;
     std::vector<float> _f;
     std::vector<uint8_t> _u8;
+};
+
+// rgb: 3 channels (or null), alpha: 1 channel (or null) (cpu/image.h:95-120)
+struct RGBAImage {
+    std::shared_ptr<Image> rgb, alpha;
 };
 
 // ---------------------------------------------------------------- textures (cpu/texture.h)
@@ -304,6 +314,11 @@ class TriangleMesh : public Object {
     std::vector<int32_t> shape_offsets;               // num_shapes + 1
     std::vector<Material> materials;
     int num_faces() const { return (int)face_material.size(); }
+    // Wavefront OBJ + MTL (obj_loader.cpp), as TriangleMesh::from_obj maps them
+    // (object.cpp:454-611); `material` (or a white Lambertian) serves faces
+    // without a material.
+    static std::shared_ptr<TriangleMesh> from_obj(const std::string &path,
+                                                  const std::shared_ptr<Material> &material = nullptr);
 };
 
 // ---------------------------------------------------------------- lights (cpu/light.h)
