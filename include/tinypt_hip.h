@@ -289,6 +289,26 @@ int tpt_hip_cache_eval(tpt_hip_scene *scene, const uint16_t *inputs, int32_t n, 
 int tpt_hip_cache_train(tpt_hip_scene *scene, const uint16_t *inputs, const float *targets, int32_t n, float lr,
                         float *loss, float *grad);
 
+/* ---- progressive sessions ------------------------------------------------
+ * No counterpart in the reference, whose PathTracer::render (path_tracer.h:26-44)
+ * is one synchronous call.  A session accumulates frames of one sample per pixel
+ * on the device.  Frame f is sample f of the per-(pixel, sample) streams of
+ * params->seed, so F frames added over any number of calls give the image one
+ * tpt_hip_render_cached call of F frames gives (bit for bit without the cache;
+ * with it, up to the order of the gradient sums).  `cache` may be NULL (plain
+ * progressive rendering; its `frames` field is not used).  add only enqueues
+ * the frames on the scene's stream and returns; wait and read block until they
+ * are done; read writes the HDR mean of the frames so far (row 0 at the bottom).
+ * The session owns its path pool; with a cache it trains the scene handle's
+ * network.  Calls on one scene handle are ordered on its stream. */
+typedef struct tpt_hip_progress tpt_hip_progress;
+int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *params, const tpt_cache_params *cache,
+                            tpt_hip_progress **out);
+int tpt_hip_progress_add(tpt_hip_progress *session, int32_t frames);
+int tpt_hip_progress_wait(tpt_hip_progress *session);
+int tpt_hip_progress_read(tpt_hip_progress *session, float *out_rgb, int32_t *frames);
+void tpt_hip_progress_destroy(tpt_hip_progress *session);
+
 /* Closest hit for n rays of 8 floats (org.xyz, tmin, dir.xyz, tmax); directions
  * must be unit length as in the reference.  Writes the global primitive id
  * (-1 on miss) and hit distance (INF on miss).  Geometric test only (no alpha

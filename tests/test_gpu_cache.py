@@ -146,3 +146,41 @@ def test_cached_render_trains_online(hip, suffix_len):
     # the network keeps training across calls
     _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, suffix_len=suffix_len, **kw)
     assert st2["steps"] == 64 * 4 + 2 * 4
+
+
+def test_progressive_session_splits_equal_one_render(hip):
+    # frames added over several calls continue the same streams: 3 + 5 frames
+    # equal one 8-frame progressive render (and the plain 8-spp HDR render)
+    from tinypt_amd import scenes
+
+    dev = scenes.triangle_room(64, 48).to("hip")
+    kw = dict(max_depth=6, rr_depth=3, seed=9)
+    one, _ = hip.render_cached(dev, 8, cache_depth=-1, **kw)
+    sess = hip.ProgressiveRender(dev, **kw)
+    sess.add(3)  # enqueued, returns at once
+    mid, n = sess.read()
+    assert n == 3 and np.all(np.isfinite(mid))
+    sess.add(2).add(3)
+    img, n = sess.read()
+    assert n == 8
+    assert np.array_equal(img, one)
+    plain = hip.PathTracer("hip").render(dev, 8, hdr=True, **kw).raw()
+    assert np.array_equal(img, plain)
+    sess.close()
+
+
+def test_progressive_session_with_cache(hip):
+    from tinypt_amd import scenes
+
+    dev = scenes.triangle_room(128, 96).to("hip")
+    kw = dict(max_depth=8, rr_depth=5, seed=1)
+    ref = hip.PathTracer("hip").render(dev, 1024, hdr=True, fp64=True, max_depth=8, rr_depth=5, seed=99).raw()
+    sess = hip.ProgressiveRender(dev, cache_depth=2, train_records=2048, reset=True, weight_seed=7, **kw)
+    for _ in range(4):
+        sess.add(16)
+    sess.wait()
+    img, n = sess.read()
+    assert n == 64 and np.all(np.isfinite(img)) and img.min() >= 0
+    assert abs(img.mean() - ref.mean()) < 0.05 * ref.mean()
+    with pytest.raises(Exception):
+        sess.add(0)

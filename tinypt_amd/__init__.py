@@ -132,6 +132,11 @@ C_ABI_SYMBOLS = (
     "tpt_hip_cache_weights",
     "tpt_hip_cache_eval",
     "tpt_hip_cache_train",
+    "tpt_hip_progress_create",
+    "tpt_hip_progress_add",
+    "tpt_hip_progress_wait",
+    "tpt_hip_progress_read",
+    "tpt_hip_progress_destroy",
 )
 
 
@@ -161,8 +166,15 @@ def lib():
     L.tpt_hip_cache_weights.argtypes = [vp, vp, vp]
     L.tpt_hip_cache_eval.argtypes = [vp, vp, i32, vp]
     L.tpt_hip_cache_train.argtypes = [vp, vp, vp, i32, ctypes.c_float, vp, vp]
+    L.tpt_hip_progress_create.argtypes = [vp, ctypes.POINTER(_RenderParams), ctypes.POINTER(_CacheParams), ctypes.POINTER(vp)]
+    L.tpt_hip_progress_add.argtypes = [vp, i32]
+    L.tpt_hip_progress_wait.argtypes = [vp]
+    L.tpt_hip_progress_read.argtypes = [vp, vp, ctypes.POINTER(i32)]
+    L.tpt_hip_progress_destroy.argtypes = [vp]
+    L.tpt_hip_progress_destroy.restype = None
     for name in C_ABI_SYMBOLS:
-        if name not in ("tpt_hip_abi_version", "tpt_hip_device_count", "tpt_hip_last_error", "tpt_hip_scene_destroy"):
+        if name not in ("tpt_hip_abi_version", "tpt_hip_device_count", "tpt_hip_last_error", "tpt_hip_scene_destroy",
+                        "tpt_hip_progress_destroy"):
             getattr(L, name).restype = ctypes.c_int
     if L.tpt_hip_abi_version() != ABI_VERSION:
         raise RuntimeError(f"{LIB_PATH} has ABI version {L.tpt_hip_abi_version()}, this package expects "
@@ -242,6 +254,55 @@ def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learn
     _check(lib().tpt_hip_render_cached(scene_hip._hip_ptr(), ctypes.byref(p), ctypes.byref(c), out.ctypes.data,
                                        ctypes.byref(st)))
     return out, {f: getattr(st, f) for f, _ in _CacheStats._fields_ if f != "pad_"}
+
+
+class ProgressiveRender:
+    """Progressive render session (tpt_hip_progress_*): frames of one sample per
+    pixel accumulate on the device across calls; add() enqueues and returns at
+    once, read() waits and returns (H x W x 3 float32 HDR mean, frames so far).
+    cache_depth >= 0 enables the learned radiance cache (arguments as in
+    render_cached).  Keep the scene alive while the session is."""
+
+    def __init__(self, scene_hip, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False,
+                 cache_depth=-1, train_records=1 << 16, learning_rate=1e-3, reset=False, weight_seed=0,
+                 train_steps=4, suffix_len=3):
+        self._scene = scene_hip
+        cam = scene_hip.camera()
+        self._shape = (cam.height, cam.width, 3)
+        p = _RenderParams(spp=1, max_depth=max_depth, rr_depth=rr_depth, shadow_bounded=int(shadow_bounded),
+                          seed=seed, output=1, accum_fp64=int(fp64))
+        c = None
+        if cache_depth >= 0:
+            c = _CacheParams(frames=1, cache_depth=cache_depth, train_records=train_records, reset=int(reset),
+                             learning_rate=learning_rate, train_steps=train_steps, weight_seed=weight_seed,
+                             suffix_len=suffix_len)
+        h = ctypes.c_void_p()
+        _check(lib().tpt_hip_progress_create(scene_hip._hip_ptr(), ctypes.byref(p),
+                                             None if c is None else ctypes.byref(c), ctypes.byref(h)))
+        self._h = h
+
+    def add(self, frames):
+        _check(lib().tpt_hip_progress_add(self._h, int(frames)))
+        return self
+
+    def wait(self):
+        _check(lib().tpt_hip_progress_wait(self._h))
+
+    def read(self):
+        import numpy as np
+
+        out = np.empty(self._shape, dtype=np.float32)
+        n = ctypes.c_int32()
+        _check(lib().tpt_hip_progress_read(self._h, out.ctypes.data, ctypes.byref(n)))
+        return out, n.value
+
+    def close(self):
+        if self._h:
+            lib().tpt_hip_progress_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def cache_weights(scene_hip, set=None):

@@ -878,42 +878,57 @@ void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
 
 } // namespace
 
-int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_params *cp, float *out_rgb,
-                          tpt_cache_stats *stats) {
-    if (!h || !p || !cp || !out_rgb) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
-    if (cp->frames <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "frames must be positive");
-    if (cp->train_records <= 0 || cp->train_records > (1 << 20))
-        return set_err(TPT_ERR_INVALID_ARGUMENT, "train_records must be in 1 .. 2^20");
-    if (p->max_depth < 0 || p->max_depth > 1000) return set_err(TPT_ERR_INVALID_ARGUMENT, "max_depth out of range");
-    if (!(cp->learning_rate >= 0.f)) return set_err(TPT_ERR_INVALID_ARGUMENT, "learning_rate must not be negative");
-    if (cp->train_steps < 1 || cp->train_steps > cp->train_records)
-        return set_err(TPT_ERR_INVALID_ARGUMENT, "train_steps must be in 1 .. train_records");
-    if (cp->suffix_len < 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "suffix_len must not be negative");
-    try {
-        auto t0 = std::chrono::steady_clock::now();
-        HIP_OK(hipSetDevice(h->device));
-        const DScene &S = h->S;
-        const int pixels = S.cam.width * S.cam.height;
-        const bool use_cache = cp->cache_depth >= 0;
-        // one sample per pixel and frame; frame f is sample f of the plain sampler
-        // with spp = frames and no sub-pixel strata (b = 1, nsub = frames)
-        DRender R = make_render(p, 1, p->seed, pixels);
-        R.block = 1;
-        R.nsub = cp->frames;
-        R.spp_eff = cp->frames;
-        R.output = 1;
-        R.k = 1;
-        hipStream_t st = h->stream;
-        ensure_paths(h, (size_t)pixels, pixels);
-        DPaths &P = h->pb.P;
+namespace {
+
+// Progressive frames of one sample per pixel on a scene: frame f is sample f of
+// the plain sampler with one sub-pixel block (b = 1), so its camera ray and
+// stream do not depend on how many frames are rendered in total.
+struct FrameSetup {
+    DRender R{};
+    DCache C{};
+    tpt_cache_params cp{};
+    bool use_cache = false;
+    int pixels = 0, grid = 0, fp64 = 0;
+    uint64_t base_mixed = 0;
+};
+constexpr int kFrameBlock = 256;
+
+// Validates the parameters (cache may be null: no cache; its `frames` field is
+// not used) and prepares the scene's network and cache buffers.
+void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_params *cp, FrameSetup &fs) {
+    auto bad = [](const char *m) { throw tpt_error(TPT_ERR_INVALID_ARGUMENT, m); };
+    if (p->max_depth < 0 || p->max_depth > 1000) bad("max_depth out of range");
+    fs = FrameSetup{};
+    fs.use_cache = cp && cp->cache_depth >= 0;
+    if (cp) {
+        if (cp->train_records <= 0 || cp->train_records > (1 << 20)) bad("train_records must be in 1 .. 2^20");
+        if (!(cp->learning_rate >= 0.f)) bad("learning_rate must not be negative");
+        if (cp->train_steps < 1 || cp->train_steps > cp->train_records)
+            bad("train_steps must be in 1 .. train_records");
+        if (cp->suffix_len < 0) bad("suffix_len must not be negative");
+        fs.cp = *cp;
+    }
+    HIP_OK(hipSetDevice(h->device));
+    const DScene &S = h->S;
+    fs.pixels = S.cam.width * S.cam.height;
+    fs.R = make_render(p, 1, p->seed, fs.pixels);
+    fs.R.block = 1;
+    fs.R.nsub = INT32_MAX; // one sub-pixel: sample s never leaves it
+    fs.R.spp_eff = 1;
+    fs.R.output = 1;
+    fs.R.k = 1;
+    fs.grid = grid_for((size_t)fs.pixels, kFrameBlock, h->cus);
+    fs.base_mixed = splitmix64(p->seed);
+    fs.fp64 = p->accum_fp64 ? 1 : 0;
+    if (fs.use_cache) {
         cache_alloc_weights(h);
         if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
         CacheState &c = h->cache;
-        cache_alloc_buffers(h, (size_t)pixels, cp->train_records);
-        DCache C{};
+        cache_alloc_buffers(h, (size_t)fs.pixels, cp->train_records);
+        DCache &C = fs.C;
         C.depth = cp->cache_depth;
         // enough training candidates that about train_records of them reach the cache vertex
-        C.train_stride = std::max(1, (int)((int64_t)pixels * 2 / (3 * (int64_t)cp->train_records)));
+        C.train_stride = std::max(1, (int)((int64_t)fs.pixels * 2 / (3 * (int64_t)cp->train_records)));
         C.train_cap = cp->train_records;
         for (int a = 0; a < 3; a++) {
             C.lo[a] = h->bounds_lo[a];
@@ -927,55 +942,84 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         C.q_slot = c.q_slot;
         C.t_slot = c.t_slot;
         C.counters = c.counters;
-        const int block = 256;
-        const int grid = grid_for((size_t)pixels, block, h->cus);
-        const uint64_t base_mixed = splitmix64(p->seed);
-        const int fp64 = p->accum_fp64 ? 1 : 0;
-        // four events per frame (inference and training brackets), read after the last frame
-        std::vector<hipEvent_t> ev(use_cache ? 4 * (size_t)cp->frames : 0);
-        for (auto &x : ev) HIP_OK(hipEventCreate(&x));
-        double infer_ms = 0, train_ms = 0;
-        for (int f = 0; f < cp->frames; f++) {
-            hipEvent_t *e = use_cache ? &ev[4 * (size_t)f] : nullptr;
+    }
+}
+
+// Enqueues frames [f0, f0 + n) on the scene's stream, adding each frame to P's
+// per-pixel sums (frame 0 starts them).  `ev`: 4 events per frame bracketing the
+// inference and the training launches, or null.  `clear_loss`: the last frame's
+// training loss is accumulated from zero (read back by the caller).
+void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers &pb, int f0, int n, hipEvent_t *ev, bool clear_loss) {
+    const DScene &S = h->S;
+    DPaths &P = pb.P;
+    hipStream_t st = h->stream;
+    CacheState &c = h->cache;
+    DCache &C = fs.C;
+    for (int f = f0; f < f0 + n; f++) {
+        hipEvent_t *e = ev ? &ev[4 * (size_t)(f - f0)] : nullptr;
+        if (fs.use_cache) {
             C.frame = (uint32_t)(c.step + f);
             launch_cache_reset(C, st);
-            launch_generate(S, R, P, f, pixels, base_mixed, block, st);
-            int cur = 0;
-            for (int b = 0; b <= R.max_depth; b++) {
-                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
-                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, false, (uint64_t)pixels, st);
-                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, use_cache ? &C : nullptr);
-                launch_shadow(S, P, grid, block, false, (uint64_t)pixels, st);
-                cur = 1 - cur;
-            }
-            if (use_cache) {
-                bool last = f + 1 == cp->frames;
-                if (last) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
-                HIP_OK(hipEventRecord(e[0], st));
-                nrc_launch_infer(c.feat, c.q_slot, c.counters + 0, pixels, c.W.wt, c.pred, st);
-                HIP_OK(hipEventRecord(e[1], st));
-                launch_cache_apply(P, C, c.pred, pixels, st);
-                launch_cache_targets(P, C, c.target, st);
-                HIP_OK(hipEventRecord(e[2], st));
-                // train_steps Adam steps on consecutive mini-batches of the frame's records
-                const int K = cp->train_steps, mb = (cp->train_records + K - 1) / K;
-                for (int k = 0; k < K; k++) {
-                    nrc_launch_train(c.feat_t, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
-                    nrc_launch_adam(c.W, cp->learning_rate, ++c.step, st);
-                }
-                HIP_OK(hipEventRecord(e[3], st));
-            }
-            launch_frame_accumulate(P, pixels, fp64, f == 0, st);
         }
-        launch_frame_finalize(P, pixels, fp64, cp->frames, h->pb.out, st);
+        launch_generate(S, fs.R, P, f, fs.pixels, fs.base_mixed, kFrameBlock, st);
+        int cur = 0;
+        for (int b = 0; b <= fs.R.max_depth; b++) {
+            int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
+            int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+            launch_extend(S, P, q, cur, 1 - cur, fs.grid, kFrameBlock, false, (uint64_t)fs.pixels, st);
+            launch_shade(S, fs.R, P, q, nx, cur, 1 - cur, fs.grid, kFrameBlock, st, fs.use_cache ? &C : nullptr);
+            launch_shadow(S, P, fs.grid, kFrameBlock, false, (uint64_t)fs.pixels, st);
+            cur = 1 - cur;
+        }
+        if (fs.use_cache) {
+            if (clear_loss && f + 1 == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
+            if (e) HIP_OK(hipEventRecord(e[0], st));
+            nrc_launch_infer(c.feat, c.q_slot, c.counters + 0, fs.pixels, c.W.wt, c.pred, st);
+            if (e) HIP_OK(hipEventRecord(e[1], st));
+            launch_cache_apply(P, C, c.pred, fs.pixels, st);
+            launch_cache_targets(P, C, c.target, st);
+            if (e) HIP_OK(hipEventRecord(e[2], st));
+            // train_steps Adam steps on consecutive mini-batches of the frame's records
+            const int K = fs.cp.train_steps, mb = (fs.cp.train_records + K - 1) / K;
+            for (int k = 0; k < K; k++) {
+                nrc_launch_train(c.feat_t, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
+                nrc_launch_adam(c.W, fs.cp.learning_rate, ++c.step, st);
+            }
+            if (e) HIP_OK(hipEventRecord(e[3], st));
+        }
+        launch_frame_accumulate(P, fs.pixels, fs.fp64, f == 0, st);
+    }
+}
+
+} // namespace
+
+int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_params *cp, float *out_rgb,
+                          tpt_cache_stats *stats) {
+    if (!h || !p || !cp || !out_rgb) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (cp->frames <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "frames must be positive");
+    try {
+        auto t0 = std::chrono::steady_clock::now();
+        FrameSetup fs;
+        setup_frames(h, p, cp, fs);
+        const int pixels = fs.pixels;
+        ensure_paths(h, (size_t)pixels, pixels);
+        hipStream_t st = h->stream;
+        CacheState &c = h->cache;
+        // four events per frame (inference and training brackets), read after the last frame
+        std::vector<hipEvent_t> ev(fs.use_cache ? 4 * (size_t)cp->frames : 0);
+        for (auto &x : ev) HIP_OK(hipEventCreate(&x));
+        enqueue_frames(h, fs, h->pb, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
+        launch_frame_finalize(h->pb.P, pixels, fs.fp64, cp->frames, h->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
         uint32_t cnt[2] = {0, 0};
         float loss = 0.f;
-        HIP_OK(hipMemcpyAsync(cnt, c.counters, 8, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
+        if (fs.use_cache) {
+            HIP_OK(hipMemcpyAsync(cnt, c.counters, 8, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
+        }
         HIP_OK(hipStreamSynchronize(st));
+        double infer_ms = 0, train_ms = 0;
         for (size_t f = 0; 4 * f < ev.size(); f++) {
             float a = 0, b2 = 0;
             HIP_OK(hipEventElapsedTime(&a, ev[4 * f], ev[4 * f + 1]));
@@ -989,9 +1033,9 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
             stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             stats->infer_ms = infer_ms;
             stats->train_ms = train_ms;
-            stats->last_loss = use_cache ? loss / cp->train_steps : 0.0; // sum of the mini-batch means
-            stats->queries = use_cache ? cnt[0] : 0;
-            stats->records = use_cache ? std::min<uint32_t>(cnt[1], (uint32_t)cp->train_records) : 0;
+            stats->last_loss = fs.use_cache ? loss / cp->train_steps : 0.0; // sum of the mini-batch means
+            stats->queries = fs.use_cache ? cnt[0] : 0;
+            stats->records = fs.use_cache ? std::min<uint32_t>(cnt[1], (uint32_t)cp->train_records) : 0;
             stats->steps = c.step;
         }
         return TPT_OK;
@@ -1001,6 +1045,88 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         return set_err(TPT_ERR_RUNTIME, e.what());
     }
 }
+
+// ---------------------------------------------------------------- progressive sessions
+struct tpt_hip_progress {
+    tpt_hip_scene *scene = nullptr;
+    FrameSetup fs;
+    PathBuffers pb; // the session's own path pool and per-pixel sums
+    int frames = 0;
+    ~tpt_hip_progress() {
+        if (scene) (void)hipStreamSynchronize(scene->stream); // nothing queued may still use the buffers
+        if (pb.mem) (void)hipFree(pb.mem);
+    }
+};
+
+int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *params, const tpt_cache_params *cache,
+                            tpt_hip_progress **out) {
+    if (!scene || !params || !out) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    tpt_hip_progress *s = nullptr;
+    try {
+        s = new tpt_hip_progress();
+        setup_frames(scene, params, cache, s->fs);
+        s->scene = scene;
+        alloc_paths(s->pb, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+        *out = s;
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        delete s;
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        delete s;
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+int tpt_hip_progress_add(tpt_hip_progress *s, int32_t frames) {
+    if (!s) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (frames <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "frames must be positive");
+    if ((int64_t)s->frames + frames > INT32_MAX) return set_err(TPT_ERR_INVALID_ARGUMENT, "too many frames");
+    try {
+        HIP_OK(hipSetDevice(s->scene->device));
+        enqueue_frames(s->scene, s->fs, s->pb, s->frames, frames, nullptr, false);
+        HIP_OK(hipGetLastError());
+        s->frames += frames;
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+int tpt_hip_progress_wait(tpt_hip_progress *s) {
+    if (!s) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    try {
+        HIP_OK(hipSetDevice(s->scene->device));
+        HIP_OK(hipStreamSynchronize(s->scene->stream));
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    }
+}
+
+int tpt_hip_progress_read(tpt_hip_progress *s, float *out_rgb, int32_t *frames) {
+    if (!s || !out_rgb) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->frames == 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "no frames rendered yet");
+    try {
+        HIP_OK(hipSetDevice(s->scene->device));
+        hipStream_t st = s->scene->stream;
+        launch_frame_finalize(s->pb.P, s->fs.pixels, s->fs.fp64, s->frames, s->pb.out, st);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out_rgb, s->pb.out, (size_t)s->fs.pixels * 12, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (frames) *frames = s->frames;
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+void tpt_hip_progress_destroy(tpt_hip_progress *s) { delete s; }
 
 int tpt_hip_cache_weights(tpt_hip_scene *h, const float *set, float *get) {
     if (!h) return set_err(TPT_ERR_INVALID_ARGUMENT, "null scene");
