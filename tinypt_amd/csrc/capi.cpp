@@ -145,6 +145,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     size_t o_tb = put(blob, H.tris_bvh);
     size_t o_t = put(blob, H.tris);
     size_t o_ti = put(blob, H.tri_info);
+    size_t o_sr = put(blob, H.shade_rec);
     size_t o_n = put(blob, H.normals);
     size_t o_tc = put(blob, H.texcoords);
     size_t o_gl = put(blob, H.gid_local);
@@ -177,6 +178,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.tris_bvh = (const float4 *)(base + o_tb);
     S.tris = (const float4 *)(base + o_t);
     S.tri_info = (const int4 *)(base + o_ti);
+    S.shade_rec = (const float4 *)(base + o_sr);
     S.normals = (const float *)(base + o_n);
     S.texcoords = (const float *)(base + o_tc);
     S.gid_local = (const int32_t *)(base + o_gl);
@@ -652,7 +654,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         for (int i = 0; i < n; i++) {
             table[i] = scenes[i]->S;
             DScene &T = table[i];
-            to_offset(T.nodes), to_offset(T.tris_bvh), to_offset(T.tris), to_offset(T.tri_info);
+            to_offset(T.nodes), to_offset(T.tris_bvh), to_offset(T.tris), to_offset(T.tri_info), to_offset(T.shade_rec);
             to_offset(T.normals), to_offset(T.texcoords), to_offset(T.gid_local), to_offset(T.analytic);
             to_offset(T.materials), to_offset(T.images), to_offset(T.texels), to_offset(T.delta);
             to_offset(T.lights), to_offset(T.env_marg), to_offset(T.env_cond);

@@ -12,6 +12,8 @@
 //   tris_bvh   triangles in BVH leaf order, 48 B each: (a.xyz, gid) (ab.xyz, tri_id) (ac.xyz, flags)
 //   tris       the same records in triangle-id order (surface / light-pdf lookups)
 //   tri_info   2 x int4 per triangle: (material, na, nb, nc) (ta, tb, tc, mesh)
+//   shade_rec  128 B per triangle, one cache line: everything shading reads about a hit
+//              triangle (edges, material, vertex normals, texcoords), see kShadeRec
 //   normals, texcoords, gid_local, analytic, materials, images, texels, lights, env CDF
 #pragma once
 
@@ -114,11 +116,18 @@ struct DCamera {
     float pad;
 };
 
+// Shading record of a triangle, kShadeRec float4 (128 B, one L2 line):
+//   [0] ab.xyz, material   [1] ac.xyz, flags (1: vertex normals, 2: texcoords)
+//   [2] na.xyz, nb.x       [3] nb.yz, nc.xy   [4] nc.z, ta.xy, tb.x   [5] tb.y, tc.xy, 0
+//   [6], [7] unused
+static constexpr int kShadeRec = 8;
+
 struct DScene {
     const float4 *nodes;
     const float4 *tris_bvh;
     const float4 *tris;
     const int4 *tri_info;
+    const float4 *shade_rec; // kShadeRec per triangle, in triangle-id order
     const float *normals;
     const float *texcoords;
     const int32_t *gid_local; // analytic: -(idx + 1); triangle: tri_id

@@ -65,7 +65,8 @@ struct AlphaCtx {
 struct HitRec {
     float t;
     int32_t gid;
-    float u, v; // triangle barycentrics; analytic: u = into
+    float u, v;  // triangle barycentrics; analytic: u = into
+    int32_t loc; // triangle id, or -(analytic index + 1)
 };
 
 // alpha test of a triangle candidate (object.cpp:273-287)
@@ -141,7 +142,7 @@ __device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V
             if (!any && ac.on && (__float_as_int(r2.w) & 1)) {
                 if (!tri_alpha_pass(S, __float_as_int(r1.w), u, v, ac, gid)) continue;
             }
-            best = HitRec{t, gid, u, v};
+            best = HitRec{t, gid, u, v, __float_as_int(r1.w)};
             if (any) {
                 done = true;
                 return;
@@ -170,7 +171,7 @@ __device__ __forceinline__ bool analytic_tests(const DScene &S, V3 o, V3 d, cons
                 V2 uv{(atan2f(n.y, n.x) + kPI) * (.5f * kINV_PI), acosf(n.z) * kINV_PI};
                 if (alpha_at(S, S.materials[p.material].alpha, uv) < alpha_draw(ac.key, ac.depth, p.gid)) continue;
             }
-            best = HitRec{t, p.gid, into ? 1.f : 0.f, 0.f};
+            best = HitRec{t, p.gid, into ? 1.f : 0.f, 0.f, -(i + 1)};
             if (ANY) return true;
         }
     }
@@ -397,7 +398,7 @@ struct SrcQueue { // extend: ray queue of slot ids -> hit records by queue posit
         ac = AlphaCtx{ld_stream(key + slot), ld_stream(depth + slot), true};
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
-        st_stream(hit + i, make_float4(h.t, __int_as_float(h.gid), h.u, h.v));
+        st_stream(hit + i, make_float4(h.t, __int_as_float(h.loc), h.u, h.v));
     }
 };
 struct SrcShadow { // delta-light shadow rays -> add the contribution when unoccluded
@@ -768,7 +769,7 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
         float4 o4 = P.org[slot], d4 = P.dir[slot];
         AlphaCtx ac{P.key[slot], P.depth[slot], true};
         HitRec h = trace<false, COUNT, NL, LDS>(S, mk4(o4), mk4(d4), kINF, ac, nodes, tris, cache);
-        P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+        P.hit[i] = make_float4(h.t, __int_as_float(h.loc), h.u, h.v);
     }
     if (COUNT) {
         wave_count(nodes, P.counters + kCntTraffic);
@@ -778,7 +779,7 @@ __global__ void k_extend(DScene S, DPaths P, const int32_t *queue, int qsel, int
 
 // ---------------------------------------------------------------- shade
 __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRec &h, SurfHit &sh, int &mat) {
-    int32_t loc = S.gid_local[h.gid];
+    int32_t loc = h.loc;
     if (loc < 0) {
         const DAnalytic &p = S.analytic[-loc - 1];
         mat = p.material;
@@ -799,27 +800,25 @@ __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRe
         }
         return;
     }
-    // triangle (object.cpp:266-331)
-    int tri = loc;
-    int4 i0 = S.tri_info[2 * tri], i1 = S.tri_info[2 * tri + 1];
-    mat = i0.x;
+    // triangle (object.cpp:266-331), from its shading record (one cache line)
+    const float4 *sr = S.shade_rec + (size_t)kShadeRec * loc;
+    const float4 r0 = sr[0], r1 = sr[1], r2 = sr[2], r3 = sr[3], r4 = sr[4], r5 = sr[5];
+    mat = __float_as_int(r0.w);
+    const int flags = __float_as_int(r1.w);
     float u = h.u, v = h.v;
-    float4 r1 = S.tris[3 * tri + 1], r2 = S.tris[3 * tri + 2];
-    V3 ab = mk4(r1), ac = mk4(r2);
+    V3 ab = mk4(r0), ac = mk4(r1);
+    const V2 ta{r4.y, r4.z}, tb{r4.w, r5.x}, tcv{r5.y, r5.z};
     V2 uv;
-    const float *tc = S.texcoords;
-    if (i1.x >= 0) {
+    if (flags & 2) {
         float w0 = 1.f - u - v;
-        uv = {(w0 * tc[2 * i1.x] + u * tc[2 * i1.y]) + v * tc[2 * i1.z],
-              (w0 * tc[2 * i1.x + 1] + u * tc[2 * i1.y + 1]) + v * tc[2 * i1.z + 1]};
+        uv = {(w0 * ta.x + u * tb.x) + v * tcv.x, (w0 * ta.y + u * tb.y) + v * tcv.y};
     } else {
         uv = {u, v};
     }
     V3 n;
-    if (i0.y >= 0) {
-        const float *nm = S.normals;
+    if (flags & 1) {
         float w0 = 1.f - u - v;
-        V3 na = mk(nm + 3 * i0.y), nb = mk(nm + 3 * i0.z), nc = mk(nm + 3 * i0.w);
+        V3 na{r2.x, r2.y, r2.z}, nb{r2.w, r3.x, r3.y}, nc{r3.z, r3.w, r4.x};
         n = nrm(add(add(scl(w0, na), scl(u, nb)), scl(v, nc)));
     } else {
         n = nrm(cross(ac, ab));
@@ -830,9 +829,9 @@ __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRe
     const DMaterial &m = S.materials[mat];
     if (m.bump >= 0) {
         V2 dab, dac;
-        if (i1.x >= 0) {
-            dab = {tc[2 * i1.y] - tc[2 * i1.x], tc[2 * i1.y + 1] - tc[2 * i1.x + 1]};
-            dac = {tc[2 * i1.z] - tc[2 * i1.x], tc[2 * i1.z + 1] - tc[2 * i1.x + 1]};
+        if (flags & 2) {
+            dab = {tb.x - ta.x, tb.y - ta.y};
+            dac = {tcv.x - ta.x, tcv.y - ta.y};
         } else {
             dab = {1.f, 0.f};
             dac = {0.f, 1.f};
@@ -878,7 +877,7 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
     V3 &s_org = out_.s_org, &s_dir = out_.s_dir, &s_val = out_.s_val;
     float &s_tmax = out_.s_tmax;
     float4 h4 = P.hit[i];
-    HitRec h{h4.x, __float_as_int(h4.y), h4.z, h4.w};
+    HitRec h{h4.x, -1, h4.z, h4.w, __float_as_int(h4.y)}; // the path pipeline's hit records carry the local id
     V3 o = mk4(P.org[slot]), d = mk4(P.dir[slot]);
     V3 decay = mk4(P.decay[slot]);
     V3 color = mk4(P.color[slot]);
@@ -1225,6 +1224,7 @@ __device__ __forceinline__ DScene scene_at(const DBatch &B, int s) {
     S.tris_bvh = rebase(B.base, S.tris_bvh);
     S.tris = rebase(B.base, S.tris);
     S.tri_info = rebase(B.base, S.tri_info);
+    S.shade_rec = rebase(B.base, S.shade_rec);
     S.normals = rebase(B.base, S.normals);
     S.texcoords = rebase(B.base, S.texcoords);
     S.gid_local = rebase(B.base, S.gid_local);
@@ -1436,7 +1436,7 @@ __global__ void k_extend_batch_lane(DBatch B, DPaths P, int qsel) {
                                         AlphaCtx ac{P.key[slot], P.depth[slot], true};
                                         HitRec h = trace<false, false, NL, LDS>(S, mk4(o4), mk4(d4), kINF, ac, nodes,
                                                                                 tris, 0);
-                                        P.hit[i] = make_float4(h.t, __int_as_float(h.gid), h.u, h.v);
+                                        P.hit[i] = make_float4(h.t, __int_as_float(h.loc), h.u, h.v);
                                     });
 }
 

@@ -594,6 +594,34 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
         H.tri_info.resize(2, make_int4(0, -1, -1, -1));
     }
     if (H.tris_bvh.empty()) H.tris_bvh.resize(3, make_float4(0, 0, 0, 0));
+    // ---- shading records: one 128-B line per triangle with what shading reads
+    {
+        const size_t nt = H.tri_info.size() / 2;
+        H.shade_rec.assign(nt * kShadeRec, make_float4(0, 0, 0, 0));
+        for (size_t t = 0; t < nt; t++) {
+            const int4 i0 = H.tri_info[2 * t], i1 = H.tri_info[2 * t + 1];
+            const float4 ab = H.tris[3 * t + 1], ac = H.tris[3 * t + 2];
+            const int flags = (i0.y >= 0 ? 1 : 0) | (i1.x >= 0 ? 2 : 0);
+            float n[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, uv[6] = {0, 0, 0, 0, 0, 0};
+            if (flags & 1)
+                for (int c = 0; c < 3; c++) {
+                    int ni = c == 0 ? i0.y : (c == 1 ? i0.z : i0.w);
+                    for (int q = 0; q < 3; q++) n[3 * c + q] = H.normals[3 * (size_t)ni + q];
+                }
+            if (flags & 2)
+                for (int c = 0; c < 3; c++) {
+                    int ti = c == 0 ? i1.x : (c == 1 ? i1.y : i1.z);
+                    for (int q = 0; q < 2; q++) uv[2 * c + q] = H.texcoords[2 * (size_t)ti + q];
+                }
+            float4 *r = &H.shade_rec[t * kShadeRec];
+            r[0] = make_float4(ab.x, ab.y, ab.z, __int_as_float_host(i0.x));
+            r[1] = make_float4(ac.x, ac.y, ac.z, __int_as_float_host(flags));
+            r[2] = make_float4(n[0], n[1], n[2], n[3]);
+            r[3] = make_float4(n[4], n[5], n[6], n[7]);
+            r[4] = make_float4(n[8], uv[0], uv[1], uv[2]);
+            r[5] = make_float4(uv[3], uv[4], uv[5], 0.f);
+        }
+    }
     if (H.nodes.empty()) H.nodes.resize(8, make_float4(0, 0, 0, 0));
 
     // ---- object lights: reference DFS order (cpu/scene.cpp:9-19)

@@ -22,6 +22,7 @@ struct HostScene {
     int32_t root = -1;
     std::vector<float4> tris_bvh, tris;
     std::vector<int4> tri_info;
+    std::vector<float4> shade_rec; // kShadeRec float4 per triangle (dev_types.h)
     std::vector<float> normals, texcoords;
     std::vector<int32_t> gid_local;
     std::vector<DAnalytic> analytic;
