@@ -35,6 +35,7 @@ void launch_trace_closest(const DScene &, const float4 *, uint64_t, int32_t *, f
 void launch_trace_occluded(const DScene &, const float4 *, uint64_t, uint8_t *, uint32_t *, int, int, hipStream_t);
 void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool, int, int, int, int, uint64_t,
                        hipStream_t);
+void launch_batch_tail(const DBatch &, const DPaths &, uint64_t, hipStream_t);
 void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
 void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
 void launch_cache_reset(const DCache &, hipStream_t);
@@ -571,12 +572,15 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
 
 namespace {
 
-// Device state of one batch render: scene / sampler tables, segment table and
-// the shared path pool.
+// Device state of one batch render: scene / sampler tables, segment tables and
+// path pools.  With two pools, pipeline waves alternate between them (pool 0 on
+// the scene's stream, pool 1 on a second one): one wave's shading overlaps the
+// other's traversal.  Both pools add into pool 0's per-pixel sums, each wave's
+// accumulation after the previous wave's, so the images do not change.
 struct BatchRun {
-    void *meta = nullptr;
-    PathBuffers pb;
-    DBatch B{};
+    void *meta = nullptr, *meta2 = nullptr;
+    PathBuffers pb, pb2;
+    DBatch B{}, B2{};
     uint64_t total_paths = 0, total_pixels = 0;
     std::vector<uint64_t> path_off, pix_off;
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
@@ -584,61 +588,111 @@ struct BatchRun {
     int mode = 4;          // traversal kernels (DScene::persistent)
     int chunk = 384;       // queue entries per wave of the chunked kernels
     int max_lds = 65536;
-    hipStream_t st = nullptr;
+    int pools = 1;
+    hipStream_t st = nullptr, st2 = nullptr;
     ~BatchRun() {
         if (st) (void)hipStreamSynchronize(st); // nothing queued may still use the buffers
+        if (st2) (void)hipStreamSynchronize(st2);
         if (pb.mem) (void)hipFree(pb.mem);
+        if (pb2.mem) (void)hipFree(pb2.mem);
         if (meta) (void)hipFree(meta);
+        if (meta2) (void)hipFree(meta2);
     }
-    // capture one pipeline wave of (cap, block) into an executable graph
-    hipGraphExec_t capture(int cap, int block) {
-        B.cap = cap;
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch_batch_wave(B, pb.P, max_depth, layout, block, trav_block, lds_stack, mode, chunk, max_lds, num_cu,
-                          total_pixels, st);
-        HIP_OK(hipStreamEndCapture(st, &g));
-        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        HIP_OK(e);
-        return ge;
+    // the second pool: its own path state, queues, segment table and wave start
+    void add_pool(size_t slots, hipStream_t stream2) {
+        alloc_paths(pb2, slots, (size_t)total_pixels, true);
+        pb2.P.sub_f = pb.P.sub_f;
+        pb2.P.sub_d = pb.P.sub_d;
+        pb2.P.pix_f = pb.P.pix_f;
+        pb2.P.pix_d = pb.P.pix_d;
+        size_t seg_bytes = align256(4 * (size_t)kSegFields * (size_t)B.seg_stride);
+        HIP_OK(hipMalloc(&meta2, seg_bytes + 16));
+        HIP_OK(hipMemset(meta2, 0, seg_bytes + 16));
+        B2 = B;
+        B2.seg = (int32_t *)meta2;
+        B2.state = (uint64_t *)((uint8_t *)meta2 + seg_bytes);
+        B.pools = B2.pools = 2;
+        pools = 2;
+        st2 = stream2;
     }
-    // restart at the first path with zeroed accumulators (the stream is drained
-    // first: the host copy is not ordered with the non-blocking stream)
-    void reset() {
-        uint64_t st0[2] = {0, total_paths};
+    // capture the pipeline part of one wave of (cap, block), per pool, into executable graphs
+    void capture(int cap, int block, hipGraphExec_t ge[2]) {
+        B.cap = B2.cap = cap;
+        for (int k = 0; k < pools; k++) {
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launch_batch_wave(k ? B2 : B, k ? pb2.P : pb.P, max_depth, layout, block, trav_block, lds_stack, mode,
+                              chunk, max_lds, num_cu, total_pixels, st);
+            HIP_OK(hipStreamEndCapture(st, &g));
+            hipError_t e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_OK(e);
+        }
+    }
+    // restart at the first path with zeroed accumulators (the streams are drained
+    // first: the host copies are not ordered with the non-blocking streams)
+    void reset(int cap) {
         HIP_OK(hipStreamSynchronize(st));
-        HIP_OK(hipMemcpy(B.state, st0, sizeof(st0), hipMemcpyHostToDevice));
+        if (st2) HIP_OK(hipStreamSynchronize(st2));
+        uint64_t s0[2] = {0, total_paths};
+        HIP_OK(hipMemcpy(B.state, s0, sizeof(s0), hipMemcpyHostToDevice));
+        if (pools == 2) {
+            uint64_t s1[2] = {(uint64_t)cap, total_paths};
+            HIP_OK(hipMemcpy(B2.state, s1, sizeof(s1), hipMemcpyHostToDevice));
+        }
         launch_batch_zero(B, pb.P, fp64, total_pixels, st);
     }
     int waves(int cap) const { return (int)((total_paths + (uint64_t)cap - 1) / (uint64_t)cap); }
+    // enqueue waves [0, W): pipeline graph on its pool's stream, then the tail after the
+    // previous wave's tail; done[w] (if given) is recorded after wave w
+    void run(hipGraphExec_t ge[2], int W, hipEvent_t *done) {
+        std::vector<hipEvent_t> tail(W);
+        for (auto &e : tail) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (pools == 2) { // pool 1 starts after the reset on st
+            hipEvent_t e0;
+            HIP_OK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+            HIP_OK(hipEventRecord(e0, st));
+            HIP_OK(hipStreamWaitEvent(st2, e0, 0));
+            HIP_OK(hipEventDestroy(e0));
+        }
+        for (int w = 0; w < W; w++) {
+            const int k = pools == 2 ? (w & 1) : 0;
+            hipStream_t s = k ? st2 : st;
+            HIP_OK(hipGraphLaunch(ge[k], s));
+            if (pools == 2 && w > 0) HIP_OK(hipStreamWaitEvent(s, tail[w - 1], 0));
+            launch_batch_tail(k ? B2 : B, k ? pb2.P : pb.P, total_pixels, s);
+            HIP_OK(hipEventRecord(tail[w], s));
+            if (done) HIP_OK(hipEventRecord(done[w], s));
+        }
+        if (pools == 2 && W > 0) HIP_OK(hipStreamWaitEvent(st, tail[W - 1], 0)); // st continues after the last wave
+        for (auto &e : tail) HIP_OK(hipEventDestroy(e));
+    }
 };
 
-// Paths per ms of (cap, block) over the first few pipeline waves (the first
-// replay warms the caches and is not timed when more than one runs).
+// Paths per ms of (cap, block) over the first 2 x pools pipeline waves (after an
+// untimed replay of the first `pools` waves when the batch has more waves).
 double tune_trial(BatchRun &run, int cap, int block) {
-    hipGraphExec_t ge = run.capture(cap, block);
-    run.reset();
-    int reps = std::min(run.waves(cap), 3);
+    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    run.capture(cap, block, ge);
+    const int W = run.waves(cap), reps = std::min(W, 2 * run.pools);
+    if (W > reps) { // warm the caches
+        run.reset(cap);
+        run.run(ge, std::min(W, run.pools), nullptr);
+    }
+    run.reset(cap);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
-    if (reps > 1) HIP_OK(hipGraphLaunch(ge, run.st));
     HIP_OK(hipEventRecord(e0, run.st));
-    uint64_t done = reps > 1 ? (uint64_t)cap : 0, paths = 0;
-    for (int r = (reps > 1 ? 1 : 0); r < reps; r++) {
-        HIP_OK(hipGraphLaunch(ge, run.st));
-        paths += std::min<uint64_t>((uint64_t)cap, run.total_paths - done);
-        done += (uint64_t)cap;
-    }
+    run.run(ge, reps, nullptr);
     HIP_OK(hipEventRecord(e1, run.st));
     HIP_OK(hipEventSynchronize(e1));
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipGraphExecDestroy(ge);
+    for (int k = 0; k < run.pools; k++) (void)hipGraphExecDestroy(ge[k]);
+    uint64_t paths = std::min<uint64_t>((uint64_t)reps * (uint64_t)cap, run.total_paths);
     return ms > 0 ? (double)paths / (double)ms : 0.0;
 }
 
@@ -747,6 +801,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.state = (uint64_t *)(m + o_st);
         B.n = n;
         B.seg_stride = seg_stride_for(n);
+        B.pools = 1;
         // candidate capacities: 2^18 .. 2^24 path slots, never (much) more than the batch needs
         auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
         std::vector<int> caps;
@@ -760,6 +815,10 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         if (p->block_size > 0) blocks.push_back(p->block_size);
         else blocks = {64, 128, 256, 512};
         alloc_paths(run.pb, (size_t)*std::max_element(caps.begin(), caps.end()), (size_t)run.total_pixels, true);
+        if (scenes[0]->streams == 2 && run.total_paths > (uint64_t)caps[0]) { // two waves in flight
+            if (!scenes[0]->stream2) HIP_OK(hipStreamCreateWithFlags(&scenes[0]->stream2, hipStreamNonBlocking));
+            run.add_pool((size_t)*std::max_element(caps.begin(), caps.end()), scenes[0]->stream2);
+        }
         int best_cap = caps[0], best_block = blocks.size() == 1 ? blocks[0] : 256;
         double tune_ms = 0;
         if (caps.size() > 1 || blocks.size() > 1) {
@@ -784,18 +843,16 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             }
             tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
         }
-        // the render: one captured pipeline wave, replayed until every path is done
+        // the render: one captured pipeline wave per pool, replayed until every path is done
         auto tr = std::chrono::steady_clock::now();
-        hipGraphExec_t ge = run.capture(best_cap, best_block);
-        run.reset();
+        hipGraphExec_t ge[2] = {nullptr, nullptr};
+        run.capture(best_cap, best_block, ge);
+        run.reset(best_cap);
         int W = run.waves(best_cap);
         std::vector<hipEvent_t> ev(W + 1);
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
         HIP_OK(hipEventRecord(ev[0], run.st));
-        for (int w = 0; w < W; w++) {
-            HIP_OK(hipGraphLaunch(ge, run.st));
-            HIP_OK(hipEventRecord(ev[w + 1], run.st));
-        }
+        run.run(ge, W, ev.data() + 1);
         launch_batch_finalize(B, run.pb.P, run.pb.out, run.total_pixels, run.st);
         HIP_OK(hipGetLastError());
         for (int i = 0; i < n; i++)
@@ -812,7 +869,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             }
         }
         for (auto e : ev) (void)hipEventDestroy(e);
-        (void)hipGraphExecDestroy(ge);
+        for (int k = 0; k < run.pools; k++) (void)hipGraphExecDestroy(ge[k]);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->render_ms = std::chrono::duration<double, std::milli>(t1 - tr).count();

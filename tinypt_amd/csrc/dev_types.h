@@ -271,7 +271,7 @@ struct DBatch {
     int32_t n;
     int32_t cap;
     int32_t seg_stride;         // n + 2 rounded up to kSegAlign: every field starts on its own cache lines
-    int32_t pad_;
+    int32_t pools;              // pipeline waves in flight (path pools); a pool's next wave starts pools * cap later
 };
 // Segment fields are kept on separate 256-B lines: the counters that a kernel
 // bumps with atomics would otherwise share a line with the bases and counts

@@ -1308,33 +1308,45 @@ __global__ TPT_TRAVERSAL_ATTR void k_extend_batch(DBatch B, DPaths P, int qsel) 
     }
 }
 
+// Exclusive scan of get(0 .. n) by one wavefront (each lane walks a run of
+// ceil(n / 64) entries): put(i, prefix) for every i; returns the total.  The
+// per-bounce scans of the batch pipeline run as single wavefronts so that they
+// start at once even while another pipeline wave's kernels occupy every CU.
+template <class Get, class Put> __device__ __forceinline__ int wave_scan(int n, Get get, Put put) {
+    const int lane = (int)__lane_id(), per = (n + 63) / 64;
+    const int lo = min(n, lane * per), hi = min(n, lo + per);
+    int local = 0;
+    for (int i = lo; i < hi; i++) local += get(i);
+    int incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    const int total = __shfl(incl, 63);
+    int run = incl - local;
+    for (int i = lo; i < hi; i++) {
+        int v = get(i);
+        put(i, run);
+        run += v;
+    }
+    return total;
+}
+
 // Next-bounce segments: the survivors (and shadow rays) of scene s go to
 // [scan_s, scan_s + count_s) where scan is the exclusive prefix sum of this
 // bounce's per-scene ray counts (a path leaves at most one survivor and one
-// shadow ray).  One workgroup of 1024 threads, at most 1024 scenes.
+// shadow ray).  One wavefront.
 __global__ void k_batch_seg(DBatch B, int qsel) {
-    __shared__ int32_t buf[1024];
     const int cc = (qsel ? kSegQbBase : kSegQaBase) + 1;
     const int nb = qsel ? kSegQaBase : kSegQbBase;
-    int t = threadIdx.x;
-    int c = t < B.n ? TPT_SEG(B, cc, t) : 0;
-    buf[t] = c;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int v = t >= off ? buf[t - off] : 0;
-        __syncthreads();
-        buf[t] += v;
-        __syncthreads();
-    }
-    if (t < B.n) {
-        int excl = buf[t] - c;
-        TPT_SEG(B, nb, t) = excl;
-        TPT_SEG(B, nb + 1, t) = 0;
-        TPT_SEG(B, kSegShBase, t) = excl;
-        TPT_SEG(B, kSegShCnt, t) = 0;
-        TPT_SEG(B, kSegShFetch, t) = 0;
-        TPT_SEG(B, kSegFetch, t) = 0;
-    }
+    wave_scan(B.n, [&](int s) { return TPT_SEG(B, cc, s); }, [&](int s, int excl) {
+        TPT_SEG(B, nb, s) = excl;
+        TPT_SEG(B, nb + 1, s) = 0;
+        TPT_SEG(B, kSegShBase, s) = excl;
+        TPT_SEG(B, kSegShCnt, s) = 0;
+        TPT_SEG(B, kSegShFetch, s) = 0;
+        TPT_SEG(B, kSegFetch, s) = 0;
+    });
 }
 
 // Workgroups are at most 512 threads (the autotuner's range), which lets the
@@ -1459,37 +1471,15 @@ __global__ void k_shadow_batch_lane(DBatch B, DPaths P) {
 
 // Chunks of the scenes' segments for the chunked traversal kernels: the chunk
 // size ck = chunk_of(total rays, chunk, waves) goes to out[n + 1]; out[s] =
-// exclusive prefix over scenes of ceil(count_s / ck), out[n] = the total.
-// One workgroup of 1024 threads, at most 1024 scenes.
-// Inclusive scan over the workgroup; `total` gets the sum.  `buf` may be reused
-// as soon as it returns.
-__device__ __forceinline__ int block_scan_1024(int32_t *buf, int v, int &total) {
-    int t = threadIdx.x;
-    buf[t] = v;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-        int u = t >= off ? buf[t - off] : 0;
-        __syncthreads();
-        buf[t] += u;
-        __syncthreads();
-    }
-    int r = buf[t];
-    total = buf[blockDim.x - 1];
-    __syncthreads();
-    return r;
-}
+// exclusive prefix over scenes of ceil(count_s / ck), out[n] = the total.  One
+// wavefront.
 __global__ void k_batch_chunks(DBatch B, int fcnt, int fout, int chunk, int waves) {
-    __shared__ int32_t buf[1024];
-    int t = threadIdx.x;
-    int cnt = t < B.n ? TPT_SEG(B, fcnt, t) : 0;
-    int total = 0;
-    block_scan_1024(buf, cnt, total);
+    const int total = wave_scan(B.n, [&](int s) { return TPT_SEG(B, fcnt, s); }, [](int, int) {});
     const int ck = (int)chunk_of((uint32_t)total, chunk, waves);
-    int c = (cnt + ck - 1) / ck;
-    int incl = block_scan_1024(buf, c, total);
-    if (t < B.n) TPT_SEG(B, fout, t) = incl - c;
-    if (t == B.n - 1) {
-        TPT_SEG(B, fout, B.n) = incl;
+    const int nch = wave_scan(B.n, [&](int s) { return (TPT_SEG(B, fcnt, s) + ck - 1) / ck; },
+                              [&](int s, int excl) { TPT_SEG(B, fout, s) = excl; });
+    if (__lane_id() == 0) {
+        TPT_SEG(B, fout, B.n) = nch;
         TPT_SEG(B, fout, B.n + 1) = ck;
     }
 }
@@ -1550,7 +1540,7 @@ __global__ void k_accumulate_batch(DBatch B, DPaths P) {
 }
 
 __global__ void k_batch_end(DBatch B) {
-    if (threadIdx.x == 0) B.state[0] += (uint64_t)B.cap;
+    if (threadIdx.x == 0) B.state[0] += (uint64_t)B.cap * (uint64_t)B.pools;
 }
 
 __global__ void k_batch_zero(DBatch B, DPaths P, int fp64) {
@@ -1818,7 +1808,7 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
     };
     int gen_grid = (B.cap + block - 1) / block;
     int shade_grid = std::min(gen_grid, num_cu * 2048 / block);
-    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
     if (mode >= 3) {
         // chunked traversal with the pipeline's block size: at most cap / chunk + n chunks
@@ -1833,13 +1823,13 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
                     for (int b = 0; b <= max_depth; b++) {
                         int qsel = b & 1;
                         int fb = qsel ? kSegQbBase : kSegQaBase;
-                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, fb + 1, (int)kSegRayChunk,
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(64), 0, st, B, fb + 1, (int)kSegRayChunk,
                                            chunk, chunk_waves(num_cu));
                         hipLaunchKernelGGL((k_extend_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B, P,
                                            qsel);
-                        hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                        hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(64), 0, st, B, qsel);
                         hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
-                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(1024), 0, st, B, (int)kSegShCnt,
+                        hipLaunchKernelGGL(k_batch_chunks, dim3(1), dim3(64), 0, st, B, (int)kSegShCnt,
                                            (int)kSegShChunk, chunk, chunk_waves(num_cu));
                         hipLaunchKernelGGL((k_shadow_batch_chunk<NL, POST, LDS>), dim3(cg), dim3(block), sh, st, B,
                                            P);
@@ -1847,8 +1837,6 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
                 });
             });
         });
-        hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
-        hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
         return;
     }
     if (lane) {
@@ -1863,14 +1851,12 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
                     int qsel = b & 1;
                     hipLaunchKernelGGL((k_extend_batch_lane<NL, LDS>), dim3(shade_grid), dim3(block), sh, st, B, P,
                                        qsel);
-                    hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                    hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(64), 0, st, B, qsel);
                     hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
                     hipLaunchKernelGGL((k_shadow_batch_lane<NL, LDS>), dim3(shade_grid), dim3(block), sh, st, B, P);
                 }
             });
         });
-        hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
-        hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
         return;
     }
     by_layout(layout, [&](auto L) {
@@ -1884,12 +1870,16 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
             for (int b = 0; b <= max_depth; b++) {
                 int qsel = b & 1;
                 hipLaunchKernelGGL((k_extend_batch<NL, LDS>), dim3(eg), dim3(trav_block), sh, st, B, P, qsel);
-                hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(1024), 0, st, B, qsel);
+                hipLaunchKernelGGL(k_batch_seg, dim3(1), dim3(64), 0, st, B, qsel);
                 hipLaunchKernelGGL(k_shade_batch, dim3(shade_grid), dim3(block), 0, st, B, P, qsel);
                 hipLaunchKernelGGL((k_shadow_batch<NL, LDS>), dim3(sg), dim3(trav_block), sh, st, B, P);
             }
         });
     });
+}
+// The end of a pipeline wave: its samples added to the per-pixel sums (in sample
+// order), then the pool's next wave start.
+void launch_batch_tail(const DBatch &B, const DPaths &P, uint64_t total_pixels, hipStream_t st) {
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
 }
