@@ -11,8 +11,9 @@ of the extend launches) and the CPU oracle timed on a bounded sample.
 
 Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
-                      16-256 spp) through PathTracer.render_batch: one shared
-                      path pool, one captured pipeline wave replayed, autotuned
+                      16-256 spp) through PathTracer.render_batch: two path
+                      pools on two streams, a captured pipeline wave per pool
+                      replayed, autotuned
                       queue capacity and workgroup size; per-scene completion
                       times and the same scenes rendered one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
@@ -309,7 +310,7 @@ def bench_batch(args):
     out = base_line(paths / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
         "workload": (f"batch_{len(devs)}_rooms_{ranges['tris'][0] // 1000}k-{ranges['tris'][1] // 1000}k_tris_"
                      f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8_rr5"),
-        "tracer": "plain HIP wavefront, one pool, captured pipeline wave replayed",
+        "tracer": "plain HIP wavefront, two path pools on two streams, captured pipeline wave per pool replayed",
         "paths_per_step": paths,
         "queue_capacity": cap,
         "block_size": block,

@@ -87,22 +87,29 @@ struct PathBuffers {
     float *out = nullptr;
 };
 
-// Radiance-cache state of a scene handle: the network (weights and Adam state,
-// kept across renders: the cache trains online) and the per-slot staging buffers.
-struct CacheState {
-    void *wmem = nullptr; // weights, moments, gradient, BF16 copies, loss, counters
-    NrcWeights W{};
-    uint32_t *counters = nullptr;
-    int step = 0;
+// Per-slot staging buffers of one cached frame in flight (one set per path pool).
+struct CacheBufs {
     void *bmem = nullptr; // per-slot buffers for `slots` paths
     size_t slots = 0;
     int train_cap = 0;
     uint16_t *feat = nullptr, *feat_t = nullptr;
     float4 *rec_T = nullptr, *rec_C = nullptr, *pred = nullptr, *target = nullptr;
     int32_t *q_slot = nullptr, *t_slot = nullptr;
+    uint32_t *counters = nullptr; // [0] queries, [1] training records
+    ~CacheBufs() {
+        if (bmem) (void)hipFree(bmem);
+    }
+};
+
+// Radiance-cache state of a scene handle: the network (weights and Adam state,
+// kept across renders: the cache trains online) and the staging buffers.
+struct CacheState {
+    void *wmem = nullptr; // weights, moments, gradient, BF16 copies, loss
+    NrcWeights W{};
+    int step = 0;
+    CacheBufs buf[2];
     ~CacheState() {
         if (wmem) (void)hipFree(wmem);
-        if (bmem) (void)hipFree(bmem);
     }
 };
 
@@ -332,10 +339,22 @@ void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
     alloc_paths(pb, slots, (size_t)pixels, false);
 }
 
+void ensure_stream2(tpt_hip_scene *h) {
+    if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+}
+
+// A second path pool adds into the first pool's per-pixel sums.
+void share_sums(PathBuffers &dst, const PathBuffers &src) {
+    dst.P.sub_f = src.P.sub_f;
+    dst.P.sub_d = src.P.sub_d;
+    dst.P.pix_f = src.P.pix_f;
+    dst.P.pix_d = src.P.pix_d;
+}
+
 // Second path pool and stream for two-stream renders: its own path state and
 // queues, sharing pb's per-pixel accumulators (call after ensure_paths).
 void ensure_paths2(tpt_hip_scene *h, size_t slots, int pixels) {
-    if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    ensure_stream2(h);
     PathBuffers &pb = h->pb2;
     if (!(pb.mem && pb.slots >= slots && pb.pixels == pixels)) {
         if (pb.mem) {
@@ -345,10 +364,7 @@ void ensure_paths2(tpt_hip_scene *h, size_t slots, int pixels) {
         }
         alloc_paths(pb, slots, (size_t)pixels, false);
     }
-    pb.P.sub_f = h->pb.P.sub_f;
-    pb.P.sub_d = h->pb.P.sub_d;
-    pb.P.pix_f = h->pb.P.pix_f;
-    pb.P.pix_d = h->pb.P.pix_d;
+    share_sums(pb, h->pb);
 }
 
 int block_size_for(int spp) { // PixelSampler ctor (cpu/path_tracer.cpp:10-21)
@@ -901,8 +917,7 @@ void cache_alloc_weights(tpt_hip_scene *h) {
         return o;
     };
     size_t o_w = take(4 * kNrcParams), o_m = take(4 * kNrcParams), o_v = take(4 * kNrcParams),
-           o_g = take(4 * kNrcParams), o_wt = take(2 * kNrcParams), o_wb = take(2 * kNrcParams), o_l = take(16),
-           o_c = take(16);
+           o_g = take(4 * kNrcParams), o_wt = take(2 * kNrcParams), o_wb = take(2 * kNrcParams), o_l = take(16);
     HIP_OK(hipMalloc(&c.wmem, off));
     HIP_OK(hipMemset(c.wmem, 0, off));
     uint8_t *b = (uint8_t *)c.wmem;
@@ -913,7 +928,6 @@ void cache_alloc_weights(tpt_hip_scene *h) {
     c.W.wt = (uint16_t *)(b + o_wt);
     c.W.wb = (uint16_t *)(b + o_wb);
     c.W.loss = (float *)(b + o_l);
-    c.counters = (uint32_t *)(b + o_c);
 }
 
 // Uploads FP32 weights [l][in][out] with their BF16 copies and clears the Adam state.
@@ -955,12 +969,14 @@ void cache_init_weights(tpt_hip_scene *h, uint64_t seed) {
     cache_set_weights(h, w.data());
 }
 
-// Per-slot staging buffers for `slots` paths and `train_cap` training records.
-void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
-    CacheState &c = h->cache;
+// Staging buffer set `lane` (one per path pool) for `slots` paths and
+// `train_cap` training records.
+void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap, int lane) {
+    CacheBufs &c = h->cache.buf[lane];
     if (c.bmem && c.slots >= slots && c.train_cap >= train_cap) return;
     if (c.bmem) {
         HIP_OK(hipStreamSynchronize(h->stream));
+        if (h->stream2) HIP_OK(hipStreamSynchronize(h->stream2));
         HIP_OK(hipFree(c.bmem));
         c.bmem = nullptr;
     }
@@ -972,9 +988,11 @@ void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
     };
     size_t o_f = take(slots * kNrcInputs * 2), o_ft = take(slots * kNrcInputs * 2), o_T = take(slots * 16),
            o_C = take(slots * 16), o_p = take(slots * 16),
-           o_q = take(slots * 4), o_t = take((size_t)train_cap * 4), o_tg = take((size_t)train_cap * 16);
+           o_q = take(slots * 4), o_t = take((size_t)train_cap * 4), o_tg = take((size_t)train_cap * 16),
+           o_c = take(16);
     HIP_OK(hipMalloc(&c.bmem, off));
     uint8_t *b = (uint8_t *)c.bmem;
+    HIP_OK(hipMemset(b + o_c, 0, 16));
     c.feat = (uint16_t *)(b + o_f);
     c.feat_t = (uint16_t *)(b + o_ft);
     c.rec_T = (float4 *)(b + o_T);
@@ -983,6 +1001,7 @@ void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap) {
     c.q_slot = (int32_t *)(b + o_q);
     c.t_slot = (int32_t *)(b + o_t);
     c.target = (float4 *)(b + o_tg);
+    c.counters = (uint32_t *)(b + o_c);
     c.slots = slots;
     c.train_cap = train_cap;
 }
@@ -1000,9 +1019,32 @@ struct FrameSetup {
     tpt_cache_params cp{};
     bool use_cache = false;
     int pixels = 0, grid = 0, fp64 = 0;
+    int lanes = 1; // 2: frames alternate between two path pools and streams
     uint64_t base_mixed = 0;
 };
 constexpr int kFrameBlock = 256;
+
+// Orders frames that run on different streams: `train` is recorded after the
+// latest frame's training (the next inference reads the weights it wrote),
+// `acc` after its accumulation (the per-pixel sums are shared and added in
+// frame order), `start` where the second stream joins the first.
+struct FrameChain {
+    hipEvent_t train = nullptr, acc = nullptr, start = nullptr;
+    bool any = false; // a frame has been enqueued: later frames wait on the events
+    FrameChain() = default;
+    FrameChain(const FrameChain &) = delete;
+    FrameChain &operator=(const FrameChain &) = delete;
+    void create() {
+        if (acc) return;
+        HIP_OK(hipEventCreateWithFlags(&train, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&acc, hipEventDisableTiming));
+        HIP_OK(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+    }
+    ~FrameChain() {
+        for (hipEvent_t e : {train, acc, start})
+            if (e) (void)hipEventDestroy(e);
+    }
+};
 
 // Validates the parameters (cache may be null: no cache; its `frames` field is
 // not used) and prepares the scene's network and cache buffers.
@@ -1031,11 +1073,13 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
     fs.grid = grid_for((size_t)fs.pixels, kFrameBlock, h->cus);
     fs.base_mixed = splitmix64(p->seed);
     fs.fp64 = p->accum_fp64 ? 1 : 0;
+    fs.lanes = h->streams;
+    if (fs.lanes == 2) ensure_stream2(h);
     if (fs.use_cache) {
         cache_alloc_weights(h);
         if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
         CacheState &c = h->cache;
-        cache_alloc_buffers(h, (size_t)fs.pixels, cp->train_records);
+        for (int l = 0; l < fs.lanes; l++) cache_alloc_buffers(h, (size_t)fs.pixels, cp->train_records, l);
         DCache &C = fs.C;
         C.depth = cp->cache_depth;
         // enough training candidates that about train_records of them reach the cache vertex
@@ -1046,29 +1090,48 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
             C.inv_ext[a] = 1.f / (h->bounds_hi[a] - h->bounds_lo[a]);
         }
         C.suffix = cp->suffix_len;
-        C.feat = c.feat;
-        C.feat_t = c.feat_t;
-        C.rec_T = c.rec_T;
-        C.rec_C = c.rec_C;
-        C.q_slot = c.q_slot;
-        C.t_slot = c.t_slot;
-        C.counters = c.counters;
     }
 }
 
-// Enqueues frames [f0, f0 + n) on the scene's stream, adding each frame to P's
-// per-pixel sums (frame 0 starts them).  `ev`: 4 events per frame bracketing the
-// inference and the training launches, or null.  `clear_loss`: the last frame's
-// training loss is accumulated from zero (read back by the caller).
-void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers &pb, int f0, int n, hipEvent_t *ev, bool clear_loss) {
+// The frame parameters with the staging buffers of one path pool.
+DCache lane_cache(const DCache &C0, const CacheBufs &b) {
+    DCache C = C0;
+    C.feat = b.feat;
+    C.feat_t = b.feat_t;
+    C.rec_T = b.rec_T;
+    C.rec_C = b.rec_C;
+    C.q_slot = b.q_slot;
+    C.t_slot = b.t_slot;
+    C.counters = b.counters;
+    return C;
+}
+
+// Enqueues frames [f0, f0 + n), adding each frame to the per-pixel sums that
+// the pools share (frame 0 starts them).  Frame f runs on pool and stream
+// f % lanes; a frame's inference waits for the previous frame's training and its
+// accumulation for the previous accumulation, so the result equals one stream's.
+// `ev`: 4 events per frame bracketing the inference and the training launches,
+// or null.  `clear_loss`: the last frame's training loss is accumulated from
+// zero (read back by the caller).
+void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[2], FrameChain &ch, int f0, int n,
+                    hipEvent_t *ev, bool clear_loss) {
     const DScene &S = h->S;
-    DPaths &P = pb.P;
-    hipStream_t st = h->stream;
     CacheState &c = h->cache;
-    DCache &C = fs.C;
+    const hipStream_t streams[2] = {h->stream, h->stream2};
+    ch.create();
+    if (fs.lanes == 2) { // the second stream starts after the work already queued on the first
+        HIP_OK(hipEventRecord(ch.start, h->stream));
+        HIP_OK(hipStreamWaitEvent(h->stream2, ch.start, 0));
+    }
     for (int f = f0; f < f0 + n; f++) {
+        const int lane = fs.lanes == 2 ? (f & 1) : 0;
+        hipStream_t st = streams[lane];
+        DPaths &P = pools[lane]->P;
+        CacheBufs &cb = c.buf[lane];
         hipEvent_t *e = ev ? &ev[4 * (size_t)(f - f0)] : nullptr;
+        DCache C{};
         if (fs.use_cache) {
+            C = lane_cache(fs.C, cb);
             C.frame = (uint32_t)(c.step + f);
             launch_cache_reset(C, st);
         }
@@ -1083,22 +1146,27 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers &pb, int f0, i
             cur = 1 - cur;
         }
         if (fs.use_cache) {
+            if (ch.any) HIP_OK(hipStreamWaitEvent(st, ch.train, 0));
             if (clear_loss && f + 1 == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
             if (e) HIP_OK(hipEventRecord(e[0], st));
-            nrc_launch_infer(c.feat, c.q_slot, c.counters + 0, fs.pixels, c.W.wt, c.pred, st);
+            nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, fs.pixels, c.W.wt, cb.pred, st);
             if (e) HIP_OK(hipEventRecord(e[1], st));
-            launch_cache_apply(P, C, c.pred, fs.pixels, st);
-            launch_cache_targets(P, C, c.target, st);
+            launch_cache_apply(P, C, cb.pred, fs.pixels, st);
+            launch_cache_targets(P, C, cb.target, st);
             if (e) HIP_OK(hipEventRecord(e[2], st));
             // train_steps Adam steps on consecutive mini-batches of the frame's records
             const int K = fs.cp.train_steps, mb = (fs.cp.train_records + K - 1) / K;
             for (int k = 0; k < K; k++) {
-                nrc_launch_train(c.feat_t, c.t_slot, c.target, c.counters + 1, k * mb, mb, c.W, st);
+                nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, k * mb, mb, c.W, st);
                 nrc_launch_adam(c.W, fs.cp.learning_rate, ++c.step, st);
             }
             if (e) HIP_OK(hipEventRecord(e[3], st));
+            HIP_OK(hipEventRecord(ch.train, st));
         }
+        if (ch.any) HIP_OK(hipStreamWaitEvent(st, ch.acc, 0));
         launch_frame_accumulate(P, fs.pixels, fs.fp64, f == 0, st);
+        HIP_OK(hipEventRecord(ch.acc, st));
+        ch.any = true;
     }
 }
 
@@ -1114,22 +1182,28 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         setup_frames(h, p, cp, fs);
         const int pixels = fs.pixels;
         ensure_paths(h, (size_t)pixels, pixels);
+        if (fs.lanes == 2) ensure_paths2(h, (size_t)pixels, pixels);
+        PathBuffers *pools[2] = {&h->pb, &h->pb2};
         hipStream_t st = h->stream;
         CacheState &c = h->cache;
         // four events per frame (inference and training brackets), read after the last frame
         std::vector<hipEvent_t> ev(fs.use_cache ? 4 * (size_t)cp->frames : 0);
         for (auto &x : ev) HIP_OK(hipEventCreate(&x));
-        enqueue_frames(h, fs, h->pb, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
+        FrameChain ch;
+        enqueue_frames(h, fs, pools, ch, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
+        HIP_OK(hipStreamWaitEvent(st, ch.acc, 0)); // the last frame may have run on the second stream
         launch_frame_finalize(h->pb.P, pixels, fs.fp64, cp->frames, h->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
         uint32_t cnt[2] = {0, 0};
         float loss = 0.f;
         if (fs.use_cache) {
-            HIP_OK(hipMemcpyAsync(cnt, c.counters, 8, hipMemcpyDeviceToHost, st));
+            const CacheBufs &last = c.buf[fs.lanes == 2 ? (cp->frames - 1) & 1 : 0];
+            HIP_OK(hipMemcpyAsync(cnt, last.counters, 8, hipMemcpyDeviceToHost, st));
             HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
         }
         HIP_OK(hipStreamSynchronize(st));
+        if (fs.lanes == 2) HIP_OK(hipStreamSynchronize(h->stream2));
         double infer_ms = 0, train_ms = 0;
         for (size_t f = 0; 4 * f < ev.size(); f++) {
             float a = 0, b2 = 0;
@@ -1161,11 +1235,22 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
 struct tpt_hip_progress {
     tpt_hip_scene *scene = nullptr;
     FrameSetup fs;
-    PathBuffers pb; // the session's own path pool and per-pixel sums
+    PathBuffers pb;  // the session's own path pool and per-pixel sums
+    PathBuffers pb2; // its second pool (frames on two streams), adding into pb's sums
+    FrameChain chain;
     int frames = 0;
+    void sync() const {
+        if (!scene) return;
+        HIP_OK(hipStreamSynchronize(scene->stream));
+        if (scene->stream2) HIP_OK(hipStreamSynchronize(scene->stream2));
+    }
     ~tpt_hip_progress() {
-        if (scene) (void)hipStreamSynchronize(scene->stream); // nothing queued may still use the buffers
+        if (scene) { // nothing queued may still use the buffers
+            (void)hipStreamSynchronize(scene->stream);
+            if (scene->stream2) (void)hipStreamSynchronize(scene->stream2);
+        }
         if (pb.mem) (void)hipFree(pb.mem);
+        if (pb2.mem) (void)hipFree(pb2.mem);
     }
 };
 
@@ -1179,6 +1264,10 @@ int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *param
         setup_frames(scene, params, cache, s->fs);
         s->scene = scene;
         alloc_paths(s->pb, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+        if (s->fs.lanes == 2) {
+            alloc_paths(s->pb2, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+            share_sums(s->pb2, s->pb);
+        }
         *out = s;
         return TPT_OK;
     } catch (const tpt_error &e) {
@@ -1196,7 +1285,8 @@ int tpt_hip_progress_add(tpt_hip_progress *s, int32_t frames) {
     if ((int64_t)s->frames + frames > INT32_MAX) return set_err(TPT_ERR_INVALID_ARGUMENT, "too many frames");
     try {
         HIP_OK(hipSetDevice(s->scene->device));
-        enqueue_frames(s->scene, s->fs, s->pb, s->frames, frames, nullptr, false);
+        PathBuffers *pools[2] = {&s->pb, &s->pb2};
+        enqueue_frames(s->scene, s->fs, pools, s->chain, s->frames, frames, nullptr, false);
         HIP_OK(hipGetLastError());
         s->frames += frames;
         return TPT_OK;
@@ -1211,7 +1301,7 @@ int tpt_hip_progress_wait(tpt_hip_progress *s) {
     if (!s) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
     try {
         HIP_OK(hipSetDevice(s->scene->device));
-        HIP_OK(hipStreamSynchronize(s->scene->stream));
+        s->sync();
         return TPT_OK;
     } catch (const tpt_error &e) {
         return set_err(e.code, e.what());
@@ -1224,6 +1314,7 @@ int tpt_hip_progress_read(tpt_hip_progress *s, float *out_rgb, int32_t *frames) 
     try {
         HIP_OK(hipSetDevice(s->scene->device));
         hipStream_t st = s->scene->stream;
+        HIP_OK(hipStreamWaitEvent(st, s->chain.acc, 0)); // the last frame may have run on the second stream
         launch_frame_finalize(s->pb.P, s->fs.pixels, s->fs.fp64, s->frames, s->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, s->pb.out, (size_t)s->fs.pixels * 12, hipMemcpyDeviceToHost, st));
@@ -1260,17 +1351,17 @@ namespace {
 // Host records -> device staging (rows 0..n-1 of the feature buffer, targets).
 void cache_stage(tpt_hip_scene *h, const uint16_t *inputs, const float *targets, int n) {
     cache_alloc_weights(h);
-    cache_alloc_buffers(h, (size_t)std::max(n, 64), std::max(n, 64));
-    CacheState &c = h->cache;
+    cache_alloc_buffers(h, (size_t)std::max(n, 64), std::max(n, 64), 0);
+    CacheBufs &b = h->cache.buf[0];
     HIP_OK(hipStreamSynchronize(h->stream));
-    HIP_OK(hipMemcpy(c.feat, inputs, (size_t)n * kNrcInputs * 2, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.feat, inputs, (size_t)n * kNrcInputs * 2, hipMemcpyHostToDevice));
     if (targets) {
         std::vector<float4> t(n);
         for (int i = 0; i < n; i++) t[i] = make_float4(targets[3 * i], targets[3 * i + 1], targets[3 * i + 2], 0.f);
-        HIP_OK(hipMemcpy(c.target, t.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(b.target, t.data(), (size_t)n * 16, hipMemcpyHostToDevice));
     }
     uint32_t cnt[2] = {(uint32_t)n, (uint32_t)n};
-    HIP_OK(hipMemcpy(c.counters, cnt, 8, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.counters, cnt, 8, hipMemcpyHostToDevice));
 }
 } // namespace
 
@@ -1280,10 +1371,11 @@ int tpt_hip_cache_eval(tpt_hip_scene *h, const uint16_t *inputs, int32_t n, floa
         HIP_OK(hipSetDevice(h->device));
         cache_stage(h, inputs, nullptr, n);
         CacheState &c = h->cache;
-        nrc_launch_infer(c.feat, nullptr, c.counters, n, c.W.wt, c.pred, h->stream);
+        const CacheBufs &b = c.buf[0];
+        nrc_launch_infer(b.feat, nullptr, b.counters, n, c.W.wt, b.pred, h->stream);
         HIP_OK(hipGetLastError());
         std::vector<float4> o(n);
-        HIP_OK(hipMemcpyAsync(o.data(), c.pred, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipMemcpyAsync(o.data(), b.pred, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         for (int i = 0; i < n; i++) {
             out[3 * i] = o[i].x;
@@ -1303,9 +1395,10 @@ int tpt_hip_cache_train(tpt_hip_scene *h, const uint16_t *inputs, const float *t
         HIP_OK(hipSetDevice(h->device));
         cache_stage(h, inputs, targets, n);
         CacheState &c = h->cache;
+        const CacheBufs &b = c.buf[0];
         hipStream_t st = h->stream;
         HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
-        nrc_launch_train(c.feat, nullptr, c.target, c.counters + 1, 0, n, c.W, st);
+        nrc_launch_train(b.feat, nullptr, b.target, b.counters + 1, 0, n, c.W, st);
         HIP_OK(hipGetLastError());
         if (grad) HIP_OK(hipMemcpyAsync(grad, c.W.grad, 4 * kNrcParams, hipMemcpyDeviceToHost, st));
         if (loss) HIP_OK(hipMemcpyAsync(loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
