@@ -980,10 +980,16 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
     return out_;
 }
 
+// Extra attributes of the shade kernels (occupancy-tuning builds define it, e.g.
+// as __attribute__((amdgpu_waves_per_eu(6)))).
+#ifndef TPT_SHADE_ATTR
+#define TPT_SHADE_ATTR
+#endif
+
 // CACHE: the radiance-cache hooks are on; queries and training records are
 // appended to C's lists (at most C.train_cap training records).
 template <bool CACHE>
-__global__ void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel,
+__global__ TPT_SHADE_ATTR void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel,
                         DCache C) {
     uint32_t n = P.counters[qsel * kCntLine];
     if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[kCntFetch] = 0u; // refill counter of the next extend
@@ -1351,7 +1357,7 @@ __global__ void k_batch_seg(DBatch B, int qsel) {
 
 // Workgroups are at most 512 threads (the autotuner's range), which lets the
 // compiler budget registers for that size instead of 1024.
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 512))) void
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 512))) TPT_SHADE_ATTR void
 k_shade_batch(DBatch B, DPaths P, int qsel) {
     const int cb = qsel ? kSegQbBase : kSegQaBase, nb = qsel ? kSegQaBase : kSegQbBase;
     const int32_t *queue = qsel ? P.queue_b : P.queue_a;
