@@ -433,7 +433,7 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         // keep the traversal kernels' dynamic LDS (node cache + stack) within one workgroup's limit
         int max_shared = 0;
         HIP_OK(hipDeviceGetAttribute(&max_shared, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
-        int stack_bytes = h->S.lds_stack ? kLdsStack * h->S.trav_block * 4 : 0;
+        int stack_bytes = h->S.lds_stack ? kLdsRow * h->S.trav_block * 4 : 0;
         if (stack_bytes > max_shared) {
             h->S.lds_stack = 0;
             stack_bytes = 0;
@@ -1078,7 +1078,6 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
     if (fs.use_cache) {
         cache_alloc_weights(h);
         if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
-        CacheState &c = h->cache;
         for (int l = 0; l < fs.lanes; l++) cache_alloc_buffers(h, (size_t)fs.pixels, cp->train_records, l);
         DCache &C = fs.C;
         C.depth = cp->cache_depth;

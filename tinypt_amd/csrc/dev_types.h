@@ -44,6 +44,7 @@ static constexpr uint32_t kLeafBit = 0x80000000u;
 static constexpr int kMaxLeaf = 8;
 static constexpr int kStackSize = 96; // worst-case traversal stack entries (checked at BVH build)
 static constexpr int kLdsStack = 16;  // entries per lane kept in LDS
+static constexpr int kLdsRow = kLdsStack + 1; // LDS words per lane (odd: conflict-free rows)
 static constexpr int32_t kEmptyChild = 0x7FFFFFFF; // unused slot of a 4-wide node
 enum { kNodeBvh2 = 0, kNodeBvh4 = 1, kNodeBvh4Q = 2 };
 static constexpr int kNodeFloat4[3] = {4, 8, 4}; // float4 records per node, by layout

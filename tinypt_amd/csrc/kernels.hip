@@ -87,12 +87,14 @@ __device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u
 }
 
 // Dynamic LDS of the persistent traversal kernels: [cached top nodes: 64 B each]
-// [stack: kLdsStack * blockDim.x entries when the LDS stack is on].
+// [stack: kLdsRow * blockDim.x entries when the LDS stack is on].
 extern __shared__ int32_t tpt_lds_stack[];
 
 // Traversal stack: with LDS on, the first kLdsStack entries of every lane live
-// in LDS, entry-major ([entry][lane]), so a lane always touches its own bank
-// whatever its depth; deeper entries (rare: the host bounds the worst case by
+// in LDS, one row of kLdsRow = kLdsStack + 1 words per lane ([lane][entry]): an
+// entry's address is the lane's row plus 4 * sp (a shift and an add, no
+// multiply), and the odd row length puts the 64 lanes' entries of one depth in
+// 64 different banks.  Deeper entries (rare: the host bounds the worst case by
 // kStackSize) spill to a per-lane private array.  With LDS off the whole stack
 // is the per-lane private array (scratch accesses are lane-interleaved by the
 // hardware, so they coalesce).
@@ -106,22 +108,26 @@ typedef __attribute__((address_space(3))) v4f lds_f4;
 __device__ __forceinline__ float4 from_v4f(v4f v) { return make_float4(v.x, v.y, v.z, v.w); }
 template <bool LDS>
 struct TStack {
-    lds_i32 *lds;
-    int stride, lane;
+    lds_i32 *lds; // this lane's row
     int32_t spill[kStackSize];
     // `cached_nodes`: 64-B node records stored in front of the stack
     __device__ __forceinline__ void init(int cached_nodes) {
-        lds = (lds_i32 *)tpt_lds_stack + 16 * cached_nodes;
-        stride = blockDim.x;
-        lane = threadIdx.x;
+        lds = (lds_i32 *)tpt_lds_stack + 16 * cached_nodes + (int)threadIdx.x * kLdsRow;
     }
     __device__ __forceinline__ void set(int sp, int32_t v) {
-        if (LDS && sp < kLdsStack) lds[sp * stride + lane] = v;
+        if (LDS && sp < kLdsStack) lds[sp] = v;
         else spill[sp] = v;
     }
     __device__ __forceinline__ int32_t get(int sp) const {
-        if (LDS && sp < kLdsStack) return lds[sp * stride + lane];
+        if (LDS && sp < kLdsStack) return lds[sp];
         return spill[sp];
+    }
+    // Entries sp .. sp + 2 are in LDS: put3 may write all three unconditionally.
+    __device__ __forceinline__ bool room3(int sp) const { return LDS && sp <= kLdsStack - 3; }
+    __device__ __forceinline__ void put3(int sp, int32_t a, int32_t b, int32_t c) {
+        lds[sp] = a;
+        lds[sp + 1] = b;
+        lds[sp + 2] = c;
     }
 };
 
@@ -285,9 +291,16 @@ __device__ __forceinline__ bool visit_inner(const DScene &S, int32_t cur, const 
     TPT_CSWAP(1, 3)
     TPT_CSWAP(1, 2)
 #undef TPT_CSWAP
-    if (nh > 3) stk.set(sp++, ch[3]);
-    if (nh > 2) stk.set(sp++, ch[2]);
-    if (nh > 1) stk.set(sp++, ch[1]);
+    if (stk.room3(sp)) {
+        // branch-free push of the nh - 1 far children, farthest deepest; the
+        // words above the new top are scratch
+        stk.put3(sp, nh > 3 ? ch[3] : (nh > 2 ? ch[2] : ch[1]), nh > 3 ? ch[2] : ch[1], ch[1]);
+        sp += nh - 1;
+    } else {
+        if (nh > 3) stk.set(sp++, ch[3]);
+        if (nh > 2) stk.set(sp++, ch[2]);
+        if (nh > 1) stk.set(sp++, ch[1]);
+    }
     nearc = ch[0];
     return true;
 }
@@ -1568,7 +1581,7 @@ __global__ void k_finalize_batch(DBatch B, DPaths P, float *out) {
 }
 
 // ---------------------------------------------------------------- host-visible launchers
-static inline size_t lds_bytes(int block) { return (size_t)kLdsStack * (size_t)block * sizeof(int32_t); }
+static inline size_t lds_bytes(int block) { return (size_t)kLdsRow * (size_t)block * sizeof(int32_t); }
 
 // Persistent kernels launch exactly the resident grid: num_cu x (blocks per CU
 // the occupancy calculator allows for this kernel, block size and LDS).
@@ -1603,8 +1616,9 @@ template <class F> static void by_variant(const DScene &S, F &&f) {
 }
 // Dynamic LDS of a persistent traversal launch: cached nodes + the LDS stack.
 // LDS plan of a one-ray-per-lane launch of `block` threads: the stack's first
-// kLdsStack entries per lane when S.lds_stack is on and they fit, then as much
-// of the node cache (quantized layout) as the rest of the workgroup's LDS holds.
+// kLdsStack entries per lane (rows of kLdsRow words) when S.lds_stack is on and
+// they fit, then as much of the node cache (quantized layout) as the rest of the
+// workgroup's LDS holds.
 struct LaneLds {
     bool stack;
     int cache;
