@@ -397,24 +397,34 @@ __device__ __forceinline__ void st_stream(float4 *p, float4 v) {
 }
 template <class T> __device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
 
+// Path rays carry their closest analytic hit (put_ray): org.w = t, dir.w = the
+// analytic index * 2 + into, or -1.  Sources with kAnalyticDone start the
+// traversal from it; the others test the analytic primitives when a lane
+// takes the ray (ray_start).
 struct SrcQueue { // extend: ray queue of slot ids -> hit records by queue position
+    static constexpr bool kAnalyticDone = true;
     const int32_t *queue;
     const float4 *org, *dir;
     const uint64_t *key;
     const int32_t *depth;
     float4 *hit;
-    __device__ __forceinline__ void load(uint32_t i, V3 &o, V3 &d, float &tmax, AlphaCtx &ac) const {
+    __device__ __forceinline__ void load_hit(const DScene &S, uint32_t i, V3 &o, V3 &d, AlphaCtx &ac,
+                                             HitRec &best) const {
         int slot = ld_stream(queue + i);
-        o = mk4(ld_stream(org + slot));
-        d = mk4(ld_stream(dir + slot));
-        tmax = kINF;
+        float4 o4 = ld_stream(org + slot), d4 = ld_stream(dir + slot);
+        o = mk4(o4);
+        d = mk4(d4);
         ac = AlphaCtx{ld_stream(key + slot), ld_stream(depth + slot), true};
+        const int code = __float_as_int(d4.w);
+        best = code < 0 ? HitRec{o4.w, -1, 0.f, 0.f, 0}
+                        : HitRec{o4.w, S.analytic[code >> 1].gid, (float)(code & 1), 0.f, -((code >> 1) + 1)};
     }
     __device__ __forceinline__ void store(uint32_t i, const HitRec &h) const {
         st_stream(hit + i, make_float4(h.t, __int_as_float(h.loc), h.u, h.v));
     }
 };
 struct SrcShadow { // delta-light shadow rays -> add the contribution when unoccluded
+    static constexpr bool kAnalyticDone = false;
     const float4 *org, *dir, *val;
     const int32_t *slot;
     float4 *color;
@@ -434,6 +444,7 @@ struct SrcShadow { // delta-light shadow rays -> add the contribution when unocc
     }
 };
 struct SrcRays { // ray-query API: (org, tmin, dir, tmax) records -> prim id + t, or occlusion byte
+    static constexpr bool kAnalyticDone = false;
     const float4 *rays;
     int32_t *prim;
     float *t;
@@ -496,6 +507,23 @@ struct FetchLocal { // the wave's own range of the queue: no atomics, rays stay 
     }
 };
 
+// Loads ray i of `src` with its starting closest hit; true when the ray needs no
+// traversal (an any-hit query already occluded, or no BVH).
+template <bool ANY, class Src>
+__device__ __forceinline__ bool ray_start(const DScene &S, const Src &src, uint32_t i, V3 &o, V3 &d, AlphaCtx &ac,
+                                          HitRec &best) {
+    if constexpr (Src::kAnalyticDone) {
+        static_assert(!ANY, "precomputed analytic hits are closest hits");
+        src.load_hit(S, i, o, d, ac, best);
+        return S.root == -1;
+    } else {
+        float tmax;
+        src.load(i, o, d, tmax, ac);
+        best = HitRec{tmax, -1, 0.f, 0.f, 0};
+        return analytic_tests<ANY>(S, o, d, ac, best) || S.root == -1;
+    }
+}
+
 // Rays [.., n) from `fetch`; `cache`: LDS node-cache size (staged by the caller).
 template <bool ANY, bool COUNT, int NL, bool POSTPONE, bool LDS, class Src, class Fetch>
 __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src, uint32_t n, Fetch &fetch,
@@ -525,10 +553,7 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
             if ((need >> lane) & 1ull) {
                 uint32_t idx = base + (uint32_t)__popcll(need & lower);
                 if (idx < n) {
-                    float tmax;
-                    src.load(idx, o, d, tmax, ac);
-                    best = HitRec{tmax, -1, 0.f, 0.f};
-                    if (analytic_tests<ANY>(S, o, d, ac, best) || S.root == -1) {
+                    if (ray_start<ANY>(S, src, idx, o, d, ac, best)) {
                         src.store(idx, best);
                     } else {
                         ray = idx;
@@ -715,6 +740,18 @@ __global__ TPT_TRAVERSAL_ATTR void k_trace_pt(DScene S, SrcRays src, uint32_t n,
 
 // ---------------------------------------------------------------- generate
 // Camera ray and erand48 stream of sample `s` of pixel `pix`, written to slot j.
+// Writes a path ray with its closest analytic hit in the spare words (org.w = t,
+// dir.w = analytic index * 2 + into, or -1), tested with the alpha context the
+// extend kernels use for it (the path's key, the ray's depth).
+__device__ __forceinline__ void put_ray(const DScene &S, const DPaths &P, int slot, V3 o, V3 d, uint64_t key,
+                                        int depth) {
+    HitRec a{kINF, -1, 0.f, 0.f, 0};
+    if (S.num_analytic > 0) analytic_tests<false>(S, o, d, AlphaCtx{key, depth, true}, a);
+    const int code = a.loc < 0 ? ((-a.loc - 1) << 1) | (a.u != 0.f ? 1 : 0) : -1;
+    P.org[slot] = make_float4(o.x, o.y, o.z, a.t);
+    P.dir[slot] = make_float4(d.x, d.y, d.z, __int_as_float(code));
+}
+
 __device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, const DPaths &P, int j, int pix, int s,
                                          uint64_t base_mixed) {
     int W = S.cam.width;
@@ -741,8 +778,7 @@ __device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, cons
     V3 dir = sub(add(add(scl(a, mk(c.right)), scl(b, mk(c.up))), mk(c.direct)), offset);
     dir = nrm(dir);
     V3 org = add(mk(c.center), offset);
-    P.org[j] = make_float4(org.x, org.y, org.z, 0.f);
-    P.dir[j] = make_float4(dir.x, dir.y, dir.z, 0.f);
+    put_ray(S, P, j, org, dir, key, 0);
     P.decay[j] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.color[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     P.rng[j] = rng.x;
@@ -981,8 +1017,7 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
                 }
             }
             if (alive) {
-                P.org[slot] = make_float4(hp.x, hp.y, hp.z, 0.f);
-                P.dir[slot] = make_float4(out.x, out.y, out.z, 0.f);
+                put_ray(S, P, slot, hp, out, P.key[slot], depth + 1);
                 P.decay[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
                 P.depth[slot] = depth + 1;
             }
