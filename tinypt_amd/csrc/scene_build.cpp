@@ -57,7 +57,8 @@ struct Builder {
     std::vector<BNode> nodes;
     int max_depth = 0;
     static constexpr int kBins = 16;
-    static constexpr int kLeafTarget = 4;
+    static constexpr int kLeafTarget = 4;    // largest leaf the SAH may keep (<= kMaxLeaf)
+    static constexpr float kNodeCost = 1.f;  // a binary node's traversal cost in triangle tests
     static constexpr int kSahDepth = 32; // below: object-median splits (keeps depth < kStackSize)
 
     explicit Builder(const std::vector<BTri> &t) : tris(t), idx(t.size()) { std::iota(idx.begin(), idx.end(), 0); }
@@ -150,7 +151,7 @@ struct Builder {
         }
         float parent_area = area(lo, hi);
         if (best_axis >= 0) {
-            float split_cost = 1.f + (parent_area > 0 ? best_cost / parent_area : 0.f);
+            float split_cost = kNodeCost + (parent_area > 0 ? best_cost / parent_area : 0.f);
             if (n <= kLeafTarget && (float)n <= split_cost) return make_leaf();
         } else if (n <= kLeafTarget) {
             return make_leaf();
