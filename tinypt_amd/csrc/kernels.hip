@@ -193,6 +193,16 @@ __device__ __forceinline__ bool slab_hit(float t0x, float t1x, float t0y, float 
     return n <= fminf(f * kWiden, tw);
 }
 
+// Slab test on per-axis near / far distances already ordered by the ray's
+// direction (the same values as slab_hit's min / max of the two planes).
+__device__ __forceinline__ bool ordered_hit(float nx, float fx, float ny, float fy, float nz, float fz, float tw,
+                                           float &tnear) {
+    float n = fmaxf(fmaxf(nx, ny), fmaxf(nz, kEPS * 0.99f));
+    float f = fminf(fminf(fx, fy), fz);
+    tnear = n;
+    return n <= fminf(f * kWiden, tw);
+}
+
 // Inner-node visit: tests the node's child boxes against the ray (culling at the
 // current closest t `tw`), pushes the hit children except the nearest on the
 // stack (farthest first) and returns the nearest in `nearc`; false when no child
@@ -259,13 +269,20 @@ __device__ __forceinline__ bool visit_inner(const DScene &S, int32_t cur, const 
         ch[1] = __float_as_int(n3.y);
         ch[2] = __float_as_int(n3.z);
         ch[3] = __float_as_int(n3.w);
+        // near / far planes by the direction's signs: with inv > 0 a slab distance
+        // grows with q (a = scale * inv > 0, fma is monotone), so lo is the near
+        // plane; one select per packed word orders all four children
+        const bool sx = rp.inv.x < 0.f, sy = rp.inv.y < 0.f, sz = rp.inv.z < 0.f;
+        const uint32_t nqx = sx ? qhx : qlx, fqx = sx ? qlx : qhx;
+        const uint32_t nqy = sy ? qhy : qly, fqy = sy ? qly : qhy;
+        const uint32_t nqz = sz ? qhz : qlz, fqz = sz ? qlz : qhz;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int sh = 8 * i;
-            hb[i] = slab_hit(fmaf((float)((qlx >> sh) & 0xffu), ax, bx), fmaf((float)((qhx >> sh) & 0xffu), ax, bx),
-                             fmaf((float)((qly >> sh) & 0xffu), ay, by), fmaf((float)((qhy >> sh) & 0xffu), ay, by),
-                             fmaf((float)((qlz >> sh) & 0xffu), az, bz), fmaf((float)((qhz >> sh) & 0xffu), az, bz), tw,
-                             tn[i]);
+            hb[i] = ordered_hit(fmaf((float)((nqx >> sh) & 0xffu), ax, bx), fmaf((float)((fqx >> sh) & 0xffu), ax, bx),
+                                fmaf((float)((nqy >> sh) & 0xffu), ay, by), fmaf((float)((fqy >> sh) & 0xffu), ay, by),
+                                fmaf((float)((nqz >> sh) & 0xffu), az, bz), fmaf((float)((fqz >> sh) & 0xffu), az, bz),
+                                tw, tn[i]);
         }
     }
 #pragma unroll
