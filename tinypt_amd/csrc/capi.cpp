@@ -232,7 +232,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.lds_nodes = H.layout == kNodeBvh4Q ? std::min(H.bfs_prefix, env_int("TPT_LDS_NODES", nodes_default, 0, 2048)) : 0;
     S.num_nodes = (int32_t)(H.nodes.size() / kNodeFloat4[H.layout]);
     S.root = H.root;
-    S.num_tris = (int32_t)(H.tris_bvh.size() / 3);
+    S.num_tris = H.root == -1 ? 0 : (int32_t)(H.tris_bvh.size() / 3 - 1); // leaf-order records less the sentinel
     S.num_analytic = (int32_t)H.analytic.size();
     S.num_delta = (int32_t)H.delta.size();
     S.num_lights = (int32_t)H.lights.size();

@@ -40,12 +40,13 @@ static constexpr float kEPS = 1e-3f; // cpu/defs.h:24
 
 // child reference inside a node: >= 0 inner node index; < 0 leaf
 //   leaf = 0x80000000 | (first_tri << 3) | (count - 1), count <= 8
+// The empty slots of a 4-wide node are leaves of one zero-size sentinel triangle
+// (the last leaf-order record), which no ray hits.
 static constexpr uint32_t kLeafBit = 0x80000000u;
 static constexpr int kMaxLeaf = 8;
 static constexpr int kStackSize = 96; // worst-case traversal stack entries (checked at BVH build)
 static constexpr int kLdsStack = 16;  // entries per lane kept in LDS
 static constexpr int kLdsRow = kLdsStack + 1; // LDS words per lane (odd: conflict-free rows)
-static constexpr int32_t kEmptyChild = 0x7FFFFFFF; // unused slot of a 4-wide node
 enum { kNodeBvh2 = 0, kNodeBvh4 = 1, kNodeBvh4Q = 2 };
 static constexpr int kNodeFloat4[3] = {4, 8, 4}; // float4 records per node, by layout
 

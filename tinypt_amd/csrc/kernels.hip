@@ -285,14 +285,13 @@ __device__ __forceinline__ bool visit_inner(const DScene &S, int32_t cur, const 
                                 tw, tn[i]);
         }
     }
+    // empty slots are sentinel leaves (never hit), so no slot check; a hit's
+    // distance is at most tw <= kINF, a miss sorts last as +inf
+    if (!(hb[0] | hb[1] | hb[2] | hb[3])) return false;
+    constexpr float kMiss = __builtin_huge_valf();
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        hb[i] = hb[i] & (ch[i] != kEmptyChild);
-        tn[i] = hb[i] ? tn[i] : kINF;
-    }
-    int nh = (int)hb[0] + (int)hb[1] + (int)hb[2] + (int)hb[3];
-    if (nh == 0) return false;
-    // 5-comparator sorting network on (distance, child); misses sort last
+    for (int i = 0; i < 4; i++) tn[i] = hb[i] ? tn[i] : kMiss;
+    // 5-comparator sorting network on (distance, child)
 #define TPT_CSWAP(a, b)                                                                                                \
     if (tn[b] < tn[a]) {                                                                                               \
         float tt = tn[a];                                                                                              \
@@ -308,15 +307,16 @@ __device__ __forceinline__ bool visit_inner(const DScene &S, int32_t cur, const 
     TPT_CSWAP(1, 3)
     TPT_CSWAP(1, 2)
 #undef TPT_CSWAP
+    const bool h1 = tn[1] < kMiss, h2 = tn[2] < kMiss, h3 = tn[3] < kMiss;
     if (stk.room3(sp)) {
-        // branch-free push of the nh - 1 far children, farthest deepest; the
-        // words above the new top are scratch
-        stk.put3(sp, nh > 3 ? ch[3] : (nh > 2 ? ch[2] : ch[1]), nh > 3 ? ch[2] : ch[1], ch[1]);
-        sp += nh - 1;
+        // branch-free push of the far hit children, farthest deepest; the words
+        // above the new top are scratch
+        stk.put3(sp, h3 ? ch[3] : (h2 ? ch[2] : ch[1]), h3 ? ch[2] : ch[1], ch[1]);
+        sp += (int)h1 + (int)h2 + (int)h3;
     } else {
-        if (nh > 3) stk.set(sp++, ch[3]);
-        if (nh > 2) stk.set(sp++, ch[2]);
-        if (nh > 1) stk.set(sp++, ch[1]);
+        if (h3) stk.set(sp++, ch[3]);
+        if (h2) stk.set(sp++, ch[2]);
+        if (h1) stk.set(sp++, ch[1]);
     }
     nearc = ch[0];
     return true;

@@ -235,6 +235,9 @@ void quantize_wide_node(const float lo[3][4], const float hi[3][4], int n, const
                 qh |= (uint32_t)h << (8 * i);
             }
             if (ok) {
+                // empty slots: inverted boxes (lo 255, hi 0), which the ordered slab
+                // test misses for any origin within the scene's bounds
+                for (int i = n; i < 4; i++) ql |= 255u << (8 * i);
                 origin[a] = px;
                 scale[a] = (float)s;
                 qlo[a] = ql;
@@ -436,8 +439,13 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
             stk.push_back(B.nodes[n].right);
             stk.push_back(B.nodes[n].left);
         }
-        // triangles in leaf order
-        H.tris_bvh.resize((size_t)ntri * 3);
+        // triangles in leaf order, then one zero-size sentinel triangle: the empty
+        // slots of 4-wide nodes are leaves holding it (its test divides by a zero
+        // determinant and the NaN barycentrics reject it), so the traversal needs
+        // no empty-slot check
+        H.tris_bvh.assign((size_t)(ntri + 1) * 3, make_float4(0.f, 0.f, 0.f, 0.f));
+        H.tris_bvh[3 * (size_t)ntri] = make_float4(0.f, 0.f, 0.f, __int_as_float_host(INT32_MAX));
+        const int32_t sentinel_leaf = (int32_t)(kLeafBit | ((uint32_t)ntri << 3));
         int cursor = 0;
         std::vector<int> leaf_first(B.nodes.size(), -1);
         std::vector<int> stk2{0};
@@ -557,7 +565,7 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
                         r[i] = ref(w.children[i]);
                     } else {
                         for (int a = 0; a < 3; a++) lo[a][i] = hi[a][i] = 0.f;
-                        r[i] = kEmptyChild;
+                        r[i] = sentinel_leaf;
                     }
                 }
                 if (H.layout == kNodeBvh4Q) {
