@@ -13,9 +13,9 @@ Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
                       16-256 spp) through PathTracer.render_batch: two path
                       pools on two streams, a captured pipeline wave per pool
-                      replayed, autotuned
-                      queue capacity and workgroup size; per-scene completion
-                      times and the same scenes rendered one by one
+                      replayed, autotuned queue capacity and workgroup size;
+                      per-scene completion times and the same scenes rendered
+                      one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
   --workload city     the ~6M-triangle city (HDR sky CDF, sun, 1000 point lights),
                       1920x1080 at 1024 spp: plain tracer (reference image), batched pipeline
@@ -45,7 +45,7 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
 # FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
 # read inside this process, so the figure is the profiled one; it is reported only for that workload.
-EXTEND_TRAFFIC_KIB = {"fetch": 1230183.4, "fetch_scale": 2.0, "write": 272680.2, "write_scale": 1.0}
+EXTEND_TRAFFIC_KIB = {"fetch": 1227807.5, "fetch_scale": 2.0, "write": 260623.9, "write_scale": 1.0}
 DEFAULT_ROOM = (1024, 768, 256)
 
 
