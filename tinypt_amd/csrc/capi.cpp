@@ -645,15 +645,16 @@ struct BatchRun {
             HIP_OK(e);
         }
     }
-    // restart at the first path with zeroed accumulators (the streams are drained
-    // first: the host copies are not ordered with the non-blocking streams)
-    void reset(int cap) {
+    // restart at path `start` (a multiple of cap) with zeroed accumulators (the
+    // streams are drained first: the host copies are not ordered with the
+    // non-blocking streams)
+    void reset(int cap, uint64_t start = 0) {
         HIP_OK(hipStreamSynchronize(st));
         if (st2) HIP_OK(hipStreamSynchronize(st2));
-        uint64_t s0[2] = {0, total_paths};
+        uint64_t s0[2] = {start, total_paths};
         HIP_OK(hipMemcpy(B.state, s0, sizeof(s0), hipMemcpyHostToDevice));
         if (pools == 2) {
-            uint64_t s1[2] = {(uint64_t)cap, total_paths};
+            uint64_t s1[2] = {start + (uint64_t)cap, total_paths};
             HIP_OK(hipMemcpy(B2.state, s1, sizeof(s1), hipMemcpyHostToDevice));
         }
         launch_batch_zero(B, pb.P, fp64, total_pixels, st);
@@ -685,17 +686,22 @@ struct BatchRun {
     }
 };
 
-// Paths per ms of (cap, block) over the first 2 x pools pipeline waves (after an
-// untimed replay of the first `pools` waves when the batch has more waves).
+// Paths per ms of (cap, block) over 2 x pools pipeline waves (after an untimed
+// replay of `pools` waves when the batch has more waves).  The trial starts at
+// the wave holding the batch's middle path: scenes run shortest first, so the
+// first waves hold the small scenes, and the middle of the path range belongs
+// to the scenes that take most of the time.
 double tune_trial(BatchRun &run, int cap, int block) {
     hipGraphExec_t ge[2] = {nullptr, nullptr};
     run.capture(cap, block, ge);
     const int W = run.waves(cap), reps = std::min(W, 2 * run.pools);
+    const int first = std::max(0, std::min(W - reps, (int)(run.total_paths / 2 / (uint64_t)cap)));
+    const uint64_t start = (uint64_t)first * (uint64_t)cap;
     if (W > reps) { // warm the caches
-        run.reset(cap);
-        run.run(ge, std::min(W, run.pools), nullptr);
+        run.reset(cap, start);
+        run.run(ge, std::min(W - first, run.pools), nullptr);
     }
-    run.reset(cap);
+    run.reset(cap, start);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -708,7 +714,7 @@ double tune_trial(BatchRun &run, int cap, int block) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (int k = 0; k < run.pools; k++) (void)hipGraphExecDestroy(ge[k]);
-    uint64_t paths = std::min<uint64_t>((uint64_t)reps * (uint64_t)cap, run.total_paths);
+    uint64_t paths = std::min<uint64_t>((uint64_t)reps * (uint64_t)cap, run.total_paths - start);
     return ms > 0 ? (double)paths / (double)ms : 0.0;
 }
 
