@@ -7,6 +7,23 @@
 
 namespace tpt {
 
+// Read-only scene records at a wave-uniform address (light lists, analytic
+// primitives, batch scene tables) are read through the constant address
+// space: those loads are invariant, so they become scalar loads into SGPRs,
+// like a kernel-argument record.  Through a generic pointer they would be
+// vector loads, one per lane, repeated at every use.  With a divergent address
+// the compiler falls back to vector loads, so the result is the same either way.
+template <class T> __device__ __forceinline__ T load_invariant(const T *p) {
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "word-aligned records only");
+    typedef const __attribute__((address_space(4))) uint32_t const_u32;
+    const_u32 *src = (const_u32 *)p;
+    T v;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = src[k];
+    return v;
+}
+
 // ---------------------------------------------------------------- textures (cpu/texture.cpp)
 __device__ __forceinline__ void denorm_coords(V2 uv, int w, int h, int &x, int &y) { // texture.cpp:6-14
     float iu = uv.x - floorf(uv.x);
@@ -197,7 +214,9 @@ __device__ __forceinline__ V3 tri_sample(const DScene &S, int tri, V3 org, Rng &
     return nrm(sub(target, org));
 }
 __device__ __forceinline__ float tri_pdf(const DScene &S, int tri, V3 org, V3 dir) { // object.cpp:237-248
-    float4 r0 = S.tris[3 * tri], r1 = S.tris[3 * tri + 1], r2v = S.tris[3 * tri + 2];
+    // scalar loads when tri is wave-uniform (the light loop of lights_pdf)
+    float4 r0 = load_invariant(S.tris + 3 * tri), r1 = load_invariant(S.tris + 3 * tri + 1),
+           r2v = load_invariant(S.tris + 3 * tri + 2);
     V3 ab = mk4(r1), ac = mk4(r2v);
     float t, u, v;
     if (!tri_test(mk4(r0), ab, ac, org, dir, t, u, v)) return 0.f;
@@ -216,10 +235,10 @@ __device__ __forceinline__ V3 lights_sample(const DScene &S, V3 pos, Rng &rng) {
 __device__ __forceinline__ float lights_pdf(const DScene &S, V3 pos, V3 dir) { // LightGroup::pdf
     float val = 0.f;
     for (int i = 0; i < S.num_lights; i++) {
-        const DLight l = S.lights[i];
+        const DLight l = load_invariant(S.lights + i); // i is wave-uniform: scalar loads
         float p;
         if (l.kind == L_ENV) p = env_pdf(S, dir);
-        else if (l.kind == L_ANALYTIC) p = analytic_pdf(S.analytic[l.local], pos, dir);
+        else if (l.kind == L_ANALYTIC) p = analytic_pdf(load_invariant(S.analytic + l.local), pos, dir);
         else p = tri_pdf(S, l.local, pos, dir);
         val += p;
     }

@@ -162,7 +162,7 @@ __device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V
 template <bool ANY>
 __device__ __forceinline__ bool analytic_tests(const DScene &S, V3 o, V3 d, const AlphaCtx &ac, HitRec &best) {
     for (int i = 0; i < S.num_analytic; i++) {
-        const DAnalytic &p = S.analytic[i];
+        const DAnalytic p = load_invariant(S.analytic + i); // i is wave-uniform: scalar loads
         float t;
         bool into = true;
         if (p.type == A_SPHERE) {
@@ -1273,21 +1273,7 @@ __global__ void k_frame_finalize(DPaths P, int pixels, int fp64, int frames, flo
 template <class T> __device__ __forceinline__ T *rebase(const uint8_t *base, T *off) {
     return (T *)(base + (uintptr_t)off);
 }
-// The scene and sampler tables are read through the constant address space:
-// those loads are invariant, so with a wave-uniform index they become scalar
-// loads into SGPRs, like a kernel-argument record.  Through a generic pointer
-// they would be vector loads repeated at every use, adding a dependent memory
-// round trip in front of each scene-array access.
-template <class T> __device__ __forceinline__ T load_invariant(const T *p) {
-    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "word-aligned records only");
-    typedef const __attribute__((address_space(4))) uint32_t const_u32;
-    const_u32 *src = (const_u32 *)p;
-    T v;
-    uint32_t *dst = reinterpret_cast<uint32_t *>(&v);
-#pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / 4); k++) dst[k] = src[k];
-    return v;
-}
+// The batch scene and sampler tables are read with load_invariant (dev_scene.h).
 __device__ __forceinline__ DRender render_at(const DBatch &B, int s) { return load_invariant(B.renders + s); }
 __device__ __forceinline__ DScene scene_at(const DBatch &B, int s) {
     DScene S = load_invariant(B.scenes + s);
