@@ -393,8 +393,12 @@ DRender make_render(const tpt_render_params *p, int spp, uint64_t seed, int pixe
 
 int grid_for(size_t n, int block, int cus = 256) {
     size_t g = (n + block - 1) / block;
-    // grid-stride / persistent kernels: 8 resident blocks of 256 threads (32 waves) per CU
-    size_t cap = (size_t)cus * 2048 / block;
+    // one thread per item up to 65536 threads per CU (grid-stride beyond): the
+    // hardware starts a new wave wherever one ends, which hides the shade
+    // kernel's gather latency better than a few resident waves looping over
+    // many items (room 1024x768x256: 2048 threads per CU 627, 8192 647,
+    // uncapped 653 Mpaths/s)
+    size_t cap = (size_t)cus * 65536 / block;
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     return (int)g;

@@ -1865,7 +1865,11 @@ void launch_batch_wave(const DBatch &B, const DPaths &P, int max_depth, int layo
         return num_cu * per_cu;
     };
     int gen_grid = (B.cap + block - 1) / block;
-    int shade_grid = std::min(gen_grid, num_cu * 2048 / block);
+    // the captured shade launch has one shape for every bounce, so it is capped
+    // (8192 threads per CU) rather than sized to the wave: late bounces would
+    // dispatch mostly empty blocks (256 small rooms: 2048 / 8192 / uncapped
+    // threads per CU give 512 / 523 / 498 Mpaths/s)
+    int shade_grid = std::min(gen_grid, num_cu * 8192 / block);
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, B);
     hipLaunchKernelGGL(k_generate_batch, dim3(gen_grid), dim3(block), 0, st, B, P);
     if (mode >= 3) {
