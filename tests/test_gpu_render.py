@@ -220,3 +220,15 @@ def test_two_streams_render_identically(hip, monkeypatch, hdr):
         images.append(hip.PathTracer("hip").render(dev, 64, seed=8, max_depth=8, rr_depth=5, hdr=hdr,
                                                    queue_capacity=96 * 64 * 10).raw())
     assert np.array_equal(images[0], images[1])
+
+
+def test_smoke_entry_point():
+    # the packaged smoke check (one small render against the oracle) passes
+    import os
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import __graft_entry__
+
+    __graft_entry__.smoke()
