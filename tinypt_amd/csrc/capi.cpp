@@ -690,21 +690,15 @@ struct BatchRun {
     }
 };
 
-// Paths per ms of (cap, block) over 2 x pools pipeline waves (after an untimed
-// replay of `pools` waves when the batch has more waves).  The trial starts at
-// the wave holding the batch's middle path: scenes run shortest first, so the
-// first waves hold the small scenes, and the middle of the path range belongs
-// to the scenes that take most of the time.
-double tune_trial(BatchRun &run, int cap, int block) {
+// Paths per ms of (cap, block) over the paths [start, start + span): every
+// candidate is timed on the same paths (span is a multiple of every candidate
+// capacity, or reaches the batch's end), so the comparison does not depend on
+// which scenes a candidate's waves happen to cover.
+double tune_trial(BatchRun &run, int cap, int block, uint64_t start, uint64_t span) {
     hipGraphExec_t ge[2] = {nullptr, nullptr};
     run.capture(cap, block, ge);
-    const int W = run.waves(cap), reps = std::min(W, 2 * run.pools);
-    const int first = std::max(0, std::min(W - reps, (int)(run.total_paths / 2 / (uint64_t)cap)));
-    const uint64_t start = (uint64_t)first * (uint64_t)cap;
-    if (W > reps) { // warm the caches
-        run.reset(cap, start);
-        run.run(ge, std::min(W - first, run.pools), nullptr);
-    }
+    const uint64_t end = std::min(run.total_paths, start + span);
+    const int reps = (int)((end - start + (uint64_t)cap - 1) / (uint64_t)cap);
     run.reset(cap, start);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
@@ -828,12 +822,14 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.n = n;
         B.seg_stride = seg_stride_for(n);
         B.pools = 1;
-        // candidate capacities: 2^18 .. 2^24 path slots, never (much) more than the batch needs
+        // candidate capacities: 2^18, 2^20, .. 2^26 path slots, never (much) more than
+        // the batch needs (bigger waves amortise each wave's fixed cost: 64 mixed
+        // rooms run at 472 / 548 / 577 Mpaths/s with 2^22 / 2^24 / 2^26)
         auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
         std::vector<int> caps;
         if (p->queue_capacity > 0) caps.push_back(round_cap((uint64_t)p->queue_capacity));
         else
-            for (int e = 18; e <= 24; e++) {
+            for (int e = 18; e <= 26; e += 2) {
                 int c = round_cap(1ull << e);
                 if (caps.empty() || caps.back() != c) caps.push_back(c);
             }
@@ -850,8 +846,13 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         if (caps.size() > 1 || blocks.size() > 1) {
             auto tt = std::chrono::steady_clock::now();
             double best = -1;
+            // every trial times the same 2^27 paths around the batch's middle (the
+            // whole batch when it is smaller): scenes run shortest first, so the
+            // first paths belong to the small scenes
+            const uint64_t span = std::min<uint64_t>(run.total_paths, (uint64_t)1 << 27);
+            const uint64_t start = (run.total_paths - span) / 2;
             for (int c : caps) { // capacity first, at the middle block size
-                double r = tune_trial(run, c, best_block);
+                double r = tune_trial(run, c, best_block, start, span);
                 if (r > best) {
                     best = r;
                     best_cap = c;
@@ -860,7 +861,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             if (blocks.size() > 1) {
                 for (int b : blocks) {
                     if (b == 256) continue; // measured above
-                    double r = tune_trial(run, best_cap, b);
+                    double r = tune_trial(run, best_cap, b, start, span);
                     if (r > best) {
                         best = r;
                         best_block = b;
