@@ -225,9 +225,11 @@ typedef struct {
     int32_t pad_;
 } tpt_batch_stats;
 /* Renders n scenes (all created on the current device with the same node layout,
- * n <= 1024) through one shared path pool: one pipeline wave (generate, then
- * max_depth + 1 x extend / shade / shadow, then accumulate) is captured once in
- * a hipGraph and replayed until all paths are done.  `params` gives the shared
+ * n <= 1024) through one pipeline: a pipeline wave (generate, then
+ * max_depth + 1 x extend / shade / shadow) is captured once per path pool in a
+ * hipGraph and replayed until all paths are done, alternating between two pools
+ * on two streams (one with TPT_STREAMS=1); each wave's accumulation follows the
+ * previous wave's, so the images are bit-identical to single-scene renders.  `params` gives the shared
  * settings (max_depth, rr_depth, shadow_bounded, output, accum_fp64); its spp and
  * seed are ignored (per scene in `items`).  params->queue_capacity == 0 selects the
  * capacity by autotuning over 2^18..2^26 slots, params->block_size == 0 the
