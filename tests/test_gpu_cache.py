@@ -186,19 +186,21 @@ def test_progressive_session_with_cache(hip):
         sess.add(0)
 
 
-def test_cached_frames_on_two_streams(hip, monkeypatch):
-    """Frames alternate between two path pools and streams (the default): a frame's
-    inference waits for the previous frame's training, and the frames add into the
-    shared sums in frame order.  Without the cache, and with the cache at learning
-    rate 0 (fixed weights), the image equals the one-stream image bit for bit.
-    Training runs differ through the order in which paths append their records
-    (it splits the mini-batches), also between two runs on one stream."""
+@pytest.mark.parametrize("lanes", ["2", "3", "4"])
+def test_cached_frames_on_several_streams(hip, monkeypatch, lanes):
+    """Frames alternate between path pools on separate streams (TPT_STREAMS lanes;
+    3 by default): a frame's inference waits for the previous frame's training,
+    and the frames add into the shared sums in frame order.  Without the cache,
+    and with the cache at learning rate 0 (fixed weights), the image equals the
+    one-stream image bit for bit.  Training runs differ through the order in
+    which paths append their records (it splits the mini-batches), also between
+    two runs on one stream."""
     from tinypt_amd import scenes
 
     kw = dict(max_depth=8, rr_depth=5, seed=3)
     ck = dict(cache_depth=2, train_records=1024, reset=True, weight_seed=5, **kw)
     out = {}
-    for streams in ("1", "2"):
+    for streams in ("1", lanes):
         monkeypatch.setenv("TPT_STREAMS", streams)
         dev = scenes.triangle_room(96, 64, seed=2, tris=5000).to("hip")
         plain, _ = hip.render_cached(dev, 12, cache_depth=-1, **kw)
@@ -209,12 +211,12 @@ def test_cached_frames_on_two_streams(hip, monkeypatch):
         fixed, _ = hip.render_cached(dev, 12, learning_rate=0.0, **ck)
         cached, st = hip.render_cached(dev, 24, **ck)
         out[streams] = dict(plain=plain, split=split, fixed=fixed, cached=cached, st=st)
-    one, two = out["1"], out["2"]
-    assert np.array_equal(one["plain"], two["plain"]) and np.array_equal(one["split"], two["plain"])
-    assert np.array_equal(one["fixed"], two["fixed"])
-    assert one["st"]["steps"] == two["st"]["steps"] == 24 * 4
-    assert one["st"]["queries"] == two["st"]["queries"]
-    a, b = one["cached"], two["cached"]
+    one, many = out["1"], out[lanes]
+    assert np.array_equal(one["plain"], many["plain"]) and np.array_equal(one["split"], many["plain"])
+    assert np.array_equal(one["fixed"], many["fixed"])
+    assert one["st"]["steps"] == many["st"]["steps"] == 24 * 4
+    assert one["st"]["queries"] == many["st"]["queries"]
+    a, b = one["cached"], many["cached"]
     diff = float(np.mean(np.abs(a - b)) / a.mean())
-    print(f"cached, one vs two streams: mean |diff| / mean = {diff:.2e}")
+    print(f"cached, one vs {lanes} streams: mean |diff| / mean = {diff:.2e}")
     assert abs(a.mean() - b.mean()) < 0.02 * a.mean() and diff < 0.05

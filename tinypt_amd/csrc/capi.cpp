@@ -107,7 +107,7 @@ struct CacheState {
     void *wmem = nullptr; // weights, moments, gradient, BF16 copies, loss
     NrcWeights W{};
     int step = 0;
-    CacheBufs buf[2];
+    CacheBufs buf[4]; // one set per frame lane
     ~CacheState() {
         if (wmem) (void)hipFree(wmem);
     }
@@ -128,15 +128,24 @@ struct tpt_hip_scene {
     uint64_t last_nodes = 0, last_tris = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;      // second stream (two-stream renders), default priority
-    int streams = 2;                    // 2: renders alternate pipeline waves between two streams
+    hipStream_t streamx[2] = {nullptr, nullptr}; // third and fourth streams (frame lanes 2, 3)
+    PathBuffers pbx[2];                 // path pools of frame lanes 2, 3 (cached renders), pb's accumulators
+    int streams = 3;                    // >= 2: renders alternate pipeline waves between two streams;
+                                        // cached and progressive frames use `streams` lanes
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
     CacheState cache;
     ~tpt_hip_scene() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (stream2) (void)hipStreamSynchronize(stream2);
+        for (hipStream_t x : streamx)
+            if (x) (void)hipStreamSynchronize(x);
         if (pb.mem) (void)hipFree(pb.mem);
         if (pb2.mem) (void)hipFree(pb2.mem);
+        for (PathBuffers &x : pbx)
+            if (x.mem) (void)hipFree(x.mem);
         if (stream2) (void)hipStreamDestroy(stream2);
+        for (hipStream_t x : streamx)
+            if (x) (void)hipStreamDestroy(x);
         if (arena) (void)hipFree(arena);
         if (trace_counters) (void)hipFree(trace_counters);
         if (stream) (void)hipStreamDestroy(stream);
@@ -343,6 +352,24 @@ void ensure_stream2(tpt_hip_scene *h) {
     if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
 }
 
+// Frame lanes (cached and progressive renders): lane l runs on its own stream.
+constexpr int kMaxLanes = 4;
+hipStream_t lane_stream(const tpt_hip_scene *h, int l) {
+    return l == 0 ? h->stream : l == 1 ? h->stream2 : h->streamx[l - 2];
+}
+void ensure_lane_streams(tpt_hip_scene *h, int lanes) {
+    if (lanes > 1) ensure_stream2(h);
+    for (int l = 2; l < lanes; l++)
+        if (!h->streamx[l - 2]) HIP_OK(hipStreamCreateWithFlags(&h->streamx[l - 2], hipStreamNonBlocking));
+}
+// Waits for everything queued on the scene's streams.
+void sync_lanes(const tpt_hip_scene *h) {
+    for (int l = 0; l < kMaxLanes; l++) {
+        hipStream_t x = lane_stream(h, l);
+        if (x) HIP_OK(hipStreamSynchronize(x));
+    }
+}
+
 // A second path pool adds into the first pool's per-pixel sums.
 void share_sums(PathBuffers &dst, const PathBuffers &src) {
     dst.P.sub_f = src.P.sub_f;
@@ -446,7 +473,10 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
         // renders alternate pipeline waves between two streams (room 1024x768x256:
         // 542 vs 447 Mpaths/s, same image); TPT_STREAMS=1 keeps one
-        h->streams = env_int("TPT_STREAMS", 2, 1, 2);
+        // plain renders and batches use two pools from 2 on; cached / progressive frames
+        // use `streams` lanes (textured room 1920x1080, 16 / 64 cached frames: 54.7 / 216.9
+        // ms with 2 lanes, 50.0 / 191.9 with 3, 57.2 / 220.7 with 4)
+        h->streams = env_int("TPT_STREAMS", 3, 1, 4);
         HIP_OK(hipMalloc(&h->trace_counters, 64));
         *out = h;
         return TPT_OK;
@@ -494,7 +524,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         // the other's VALU-bound traversal.  The waves share the per-pixel sums;
         // each accumulate waits for the previous wave's, keeping sample order.
         const int nwaves = (R.spp_eff + k - 1) / k;
-        const bool two = h->streams == 2 && nwaves > 1;
+        const bool two = h->streams >= 2 && nwaves > 1;
         if (two) ensure_paths2(h, (size_t)pixels * k, pixels);
         hipStream_t st = h->stream;
         uint64_t base_mixed = splitmix64(p->seed);
@@ -837,7 +867,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         if (p->block_size > 0) blocks.push_back(p->block_size);
         else blocks = {64, 128, 256, 512};
         alloc_paths(run.pb, (size_t)*std::max_element(caps.begin(), caps.end()), (size_t)run.total_pixels, true);
-        if (scenes[0]->streams == 2 && run.total_paths > (uint64_t)caps[0]) { // two waves in flight
+        if (scenes[0]->streams >= 2 && run.total_paths > (uint64_t)caps[0]) { // two waves in flight
             if (!scenes[0]->stream2) HIP_OK(hipStreamCreateWithFlags(&scenes[0]->stream2, hipStreamNonBlocking));
             run.add_pool((size_t)*std::max_element(caps.begin(), caps.end()), scenes[0]->stream2);
         }
@@ -986,8 +1016,7 @@ void cache_alloc_buffers(tpt_hip_scene *h, size_t slots, int train_cap, int lane
     CacheBufs &c = h->cache.buf[lane];
     if (c.bmem && c.slots >= slots && c.train_cap >= train_cap) return;
     if (c.bmem) {
-        HIP_OK(hipStreamSynchronize(h->stream));
-        if (h->stream2) HIP_OK(hipStreamSynchronize(h->stream2));
+        sync_lanes(h);
         HIP_OK(hipFree(c.bmem));
         c.bmem = nullptr;
     }
@@ -1030,7 +1059,7 @@ struct FrameSetup {
     tpt_cache_params cp{};
     bool use_cache = false;
     int pixels = 0, grid = 0, fp64 = 0;
-    int lanes = 1; // 2: frames alternate between two path pools and streams
+    int lanes = 1; // frame f runs on lane f % lanes: its own path pool, stream and cache staging buffers
     uint64_t base_mixed = 0;
 };
 constexpr int kFrameBlock = 256;
@@ -1084,8 +1113,8 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
     fs.grid = grid_for((size_t)fs.pixels, kFrameBlock, h->cus);
     fs.base_mixed = splitmix64(p->seed);
     fs.fp64 = p->accum_fp64 ? 1 : 0;
-    fs.lanes = h->streams;
-    if (fs.lanes == 2) ensure_stream2(h);
+    fs.lanes = std::min(h->streams, kMaxLanes);
+    ensure_lane_streams(h, fs.lanes);
     if (fs.use_cache) {
         cache_alloc_weights(h);
         if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
@@ -1117,25 +1146,25 @@ DCache lane_cache(const DCache &C0, const CacheBufs &b) {
 }
 
 // Enqueues frames [f0, f0 + n), adding each frame to the per-pixel sums that
-// the pools share (frame 0 starts them).  Frame f runs on pool and stream
-// f % lanes; a frame's inference waits for the previous frame's training and its
-// accumulation for the previous accumulation, so the result equals one stream's.
+// the pools share (frame 0 starts them).  Frame f runs on the pool and stream
+// of lane f % lanes; a frame's inference waits for the previous frame's
+// training and its accumulation for the previous accumulation, so the result
+// equals one stream's.
 // `ev`: 4 events per frame bracketing the inference and the training launches,
 // or null.  `clear_loss`: the last frame's training loss is accumulated from
 // zero (read back by the caller).
-void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[2], FrameChain &ch, int f0, int n,
-                    hipEvent_t *ev, bool clear_loss) {
+void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[kMaxLanes], FrameChain &ch, int f0,
+                    int n, hipEvent_t *ev, bool clear_loss) {
     const DScene &S = h->S;
     CacheState &c = h->cache;
-    const hipStream_t streams[2] = {h->stream, h->stream2};
     ch.create();
-    if (fs.lanes == 2) { // the second stream starts after the work already queued on the first
+    if (fs.lanes > 1) { // the other lanes start after the work already queued on the first
         HIP_OK(hipEventRecord(ch.start, h->stream));
-        HIP_OK(hipStreamWaitEvent(h->stream2, ch.start, 0));
+        for (int l = 1; l < fs.lanes; l++) HIP_OK(hipStreamWaitEvent(lane_stream(h, l), ch.start, 0));
     }
     for (int f = f0; f < f0 + n; f++) {
-        const int lane = fs.lanes == 2 ? (f & 1) : 0;
-        hipStream_t st = streams[lane];
+        const int lane = f % fs.lanes;
+        hipStream_t st = lane_stream(h, lane);
         DPaths &P = pools[lane]->P;
         CacheBufs &cb = c.buf[lane];
         hipEvent_t *e = ev ? &ev[4 * (size_t)(f - f0)] : nullptr;
@@ -1192,8 +1221,16 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         setup_frames(h, p, cp, fs);
         const int pixels = fs.pixels;
         ensure_paths(h, (size_t)pixels, pixels);
-        if (fs.lanes == 2) ensure_paths2(h, (size_t)pixels, pixels);
-        PathBuffers *pools[2] = {&h->pb, &h->pb2};
+        if (fs.lanes > 1) ensure_paths2(h, (size_t)pixels, pixels);
+        for (int l = 2; l < fs.lanes; l++) {
+            PathBuffers &x = h->pbx[l - 2];
+            if (!(x.mem && x.slots >= (size_t)pixels && x.pixels == pixels)) {
+                sync_lanes(h);
+                alloc_paths(x, (size_t)pixels, (size_t)pixels, false);
+            }
+            share_sums(x, h->pb);
+        }
+        PathBuffers *pools[kMaxLanes] = {&h->pb, &h->pb2, &h->pbx[0], &h->pbx[1]};
         hipStream_t st = h->stream;
         CacheState &c = h->cache;
         // four events per frame (inference and training brackets), read after the last frame
@@ -1208,12 +1245,11 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         uint32_t cnt[2] = {0, 0};
         float loss = 0.f;
         if (fs.use_cache) {
-            const CacheBufs &last = c.buf[fs.lanes == 2 ? (cp->frames - 1) & 1 : 0];
+            const CacheBufs &last = c.buf[(cp->frames - 1) % fs.lanes];
             HIP_OK(hipMemcpyAsync(cnt, last.counters, 8, hipMemcpyDeviceToHost, st));
             HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
         }
-        HIP_OK(hipStreamSynchronize(st));
-        if (fs.lanes == 2) HIP_OK(hipStreamSynchronize(h->stream2));
+        sync_lanes(h);
         double infer_ms = 0, train_ms = 0;
         for (size_t f = 0; 4 * f < ev.size(); f++) {
             float a = 0, b2 = 0;
@@ -1245,22 +1281,22 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
 struct tpt_hip_progress {
     tpt_hip_scene *scene = nullptr;
     FrameSetup fs;
-    PathBuffers pb;  // the session's own path pool and per-pixel sums
-    PathBuffers pb2; // its second pool (frames on two streams), adding into pb's sums
+    PathBuffers pb;        // the session's own path pool and per-pixel sums
+    PathBuffers lanes[3];  // the pools of frame lanes 1..3, adding into pb's sums
     FrameChain chain;
     int frames = 0;
     void sync() const {
-        if (!scene) return;
-        HIP_OK(hipStreamSynchronize(scene->stream));
-        if (scene->stream2) HIP_OK(hipStreamSynchronize(scene->stream2));
+        if (scene) sync_lanes(scene);
     }
     ~tpt_hip_progress() {
-        if (scene) { // nothing queued may still use the buffers
-            (void)hipStreamSynchronize(scene->stream);
-            if (scene->stream2) (void)hipStreamSynchronize(scene->stream2);
-        }
+        if (scene) // nothing queued may still use the buffers
+            for (int l = 0; l < kMaxLanes; l++) {
+                hipStream_t x = lane_stream(scene, l);
+                if (x) (void)hipStreamSynchronize(x);
+            }
         if (pb.mem) (void)hipFree(pb.mem);
-        if (pb2.mem) (void)hipFree(pb2.mem);
+        for (PathBuffers &x : lanes)
+            if (x.mem) (void)hipFree(x.mem);
     }
 };
 
@@ -1274,9 +1310,9 @@ int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *param
         setup_frames(scene, params, cache, s->fs);
         s->scene = scene;
         alloc_paths(s->pb, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
-        if (s->fs.lanes == 2) {
-            alloc_paths(s->pb2, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
-            share_sums(s->pb2, s->pb);
+        for (int l = 1; l < s->fs.lanes; l++) {
+            alloc_paths(s->lanes[l - 1], (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+            share_sums(s->lanes[l - 1], s->pb);
         }
         *out = s;
         return TPT_OK;
@@ -1295,7 +1331,7 @@ int tpt_hip_progress_add(tpt_hip_progress *s, int32_t frames) {
     if ((int64_t)s->frames + frames > INT32_MAX) return set_err(TPT_ERR_INVALID_ARGUMENT, "too many frames");
     try {
         HIP_OK(hipSetDevice(s->scene->device));
-        PathBuffers *pools[2] = {&s->pb, &s->pb2};
+        PathBuffers *pools[kMaxLanes] = {&s->pb, &s->lanes[0], &s->lanes[1], &s->lanes[2]};
         enqueue_frames(s->scene, s->fs, pools, s->chain, s->frames, frames, nullptr, false);
         HIP_OK(hipGetLastError());
         s->frames += frames;
