@@ -205,15 +205,16 @@ def test_obj_scene_matches_oracle(hip, tmp_path):
     check(g, o, o2, 0.9)
 
 
-@pytest.mark.parametrize("hdr", [False, True])
-def test_two_streams_render_identically(hip, monkeypatch, hdr):
-    # pipeline waves alternate between two streams (the default) and share the per-pixel
-    # sums; accumulation stays in sample order, so the image equals the one-stream image
+@pytest.mark.parametrize("hdr,streams", [(False, "2"), (True, "2"), (False, "3"), (True, "4")])
+def test_several_streams_render_identically(hip, monkeypatch, hdr, streams):
+    # pipeline waves alternate between path pools on separate streams (3 by default)
+    # and share the per-pixel sums; accumulation stays in sample order, so the image
+    # equals the one-stream image
     from tinypt_amd import scenes
 
     host = scenes.triangle_room(96, 64)
     images = []
-    for streams in ("1", "2"):
+    for streams in ("1", streams):
         monkeypatch.setenv("TPT_STREAMS", streams)
         dev = host.to("hip")
         # 7 waves of 10 samples per pixel (the last one partial): both streams take several

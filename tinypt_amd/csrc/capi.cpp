@@ -130,8 +130,9 @@ struct tpt_hip_scene {
     hipStream_t stream2 = nullptr;      // second stream (two-stream renders), default priority
     hipStream_t streamx[2] = {nullptr, nullptr}; // third and fourth streams (frame lanes 2, 3)
     PathBuffers pbx[2];                 // path pools of frame lanes 2, 3 (cached renders), pb's accumulators
-    int streams = 3;                    // >= 2: renders alternate pipeline waves between two streams;
-                                        // cached and progressive frames use `streams` lanes
+    int streams = 3;                    // plain renders alternate pipeline waves, cached and progressive
+                                        // renders frames, over `streams` path pools on as many streams;
+                                        // batches use two pools from 2 on
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
     CacheState cache;
     ~tpt_hip_scene() {
@@ -362,6 +363,10 @@ void ensure_lane_streams(tpt_hip_scene *h, int lanes) {
     for (int l = 2; l < lanes; l++)
         if (!h->streamx[l - 2]) HIP_OK(hipStreamCreateWithFlags(&h->streamx[l - 2], hipStreamNonBlocking));
 }
+// Path pool of lane l of the scene (lanes >= 1 add into pool 0's per-pixel sums).
+PathBuffers *lane_pool(tpt_hip_scene *h, int l) { return l == 0 ? &h->pb : l == 1 ? &h->pb2 : &h->pbx[l - 2]; }
+void sync_lanes(const tpt_hip_scene *h);
+void ensure_lane_pool(tpt_hip_scene *h, int l, size_t slots, int pixels);
 // Waits for everything queued on the scene's streams.
 void sync_lanes(const tpt_hip_scene *h) {
     for (int l = 0; l < kMaxLanes; l++) {
@@ -392,6 +397,19 @@ void ensure_paths2(tpt_hip_scene *h, size_t slots, int pixels) {
         alloc_paths(pb, slots, (size_t)pixels, false);
     }
     share_sums(pb, h->pb);
+}
+
+// Pool of lane l with `slots` paths for `pixels` pixels (call for lane 0 first).
+void ensure_lane_pool(tpt_hip_scene *h, int l, size_t slots, int pixels) {
+    if (l == 0) return ensure_paths(h, slots, pixels);
+    if (l == 1) return ensure_paths2(h, slots, pixels);
+    ensure_lane_streams(h, l + 1);
+    PathBuffers &x = h->pbx[l - 2];
+    if (!(x.mem && x.slots >= slots && x.pixels == pixels)) {
+        sync_lanes(h);
+        alloc_paths(x, slots, (size_t)pixels, false);
+    }
+    share_sums(x, h->pb);
 }
 
 int block_size_for(int spp) { // PixelSampler ctor (cpu/path_tracer.cpp:10-21)
@@ -473,9 +491,10 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
         // renders alternate pipeline waves between two streams (room 1024x768x256:
         // 542 vs 447 Mpaths/s, same image); TPT_STREAMS=1 keeps one
-        // plain renders and batches use two pools from 2 on; cached / progressive frames
-        // use `streams` lanes (textured room 1920x1080, 16 / 64 cached frames: 54.7 / 216.9
-        // ms with 2 lanes, 50.0 / 191.9 with 3, 57.2 / 220.7 with 4)
+        // lanes of plain renders and cached / progressive frames (batches: two pools
+        // from 2 on).  Room 1024x768x256: 668 / 699 / 656 Mpaths/s with 2 / 3 / 4;
+        // textured room 1920x1080, 16 / 64 cached frames: 54.7 / 216.9 ms with 2,
+        // 50.0 / 191.9 with 3, 57.2 / 220.7 with 4
         h->streams = env_int("TPT_STREAMS", 3, 1, 4);
         HIP_OK(hipMalloc(&h->trace_counters, 64));
         *out = h;
@@ -519,13 +538,14 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 1024) return set_err(TPT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 <= 1024");
         ensure_paths(h, (size_t)pixels * k, pixels);
-        // Two streams: pipeline waves alternate between the scene's stream (pool pb)
-        // and a second one (pool pb2), so one wave's latency-bound shading overlaps
-        // the other's VALU-bound traversal.  The waves share the per-pixel sums;
-        // each accumulate waits for the previous wave's, keeping sample order.
+        // Several streams: pipeline wave w runs on lane w % npools (the scene's stream
+        // and pool pb, then stream2 / pb2, ...), so one wave's latency-bound shading
+        // overlaps another's VALU-bound traversal.  The waves share the per-pixel
+        // sums; each accumulate waits for the previous wave's, keeping sample order.
         const int nwaves = (R.spp_eff + k - 1) / k;
-        const bool two = h->streams >= 2 && nwaves > 1;
-        if (two) ensure_paths2(h, (size_t)pixels * k, pixels);
+        const int npools = std::max(1, std::min({h->streams, nwaves, kMaxLanes}));
+        const bool two = npools > 1;
+        for (int l = 1; l < npools; l++) ensure_lane_pool(h, l, (size_t)pixels * k, pixels);
         hipStream_t st = h->stream;
         uint64_t base_mixed = splitmix64(p->seed);
         bool timing = stats != nullptr;
@@ -540,15 +560,18 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             return (int)evs.size() - 1;
         };
         if (two) {
-            if (count) HIP_OK(hipMemsetAsync(h->pb2.P.counters + kCntTraffic, 0, 32, st));
-            HIP_OK(hipStreamWaitEvent(h->stream2, evs[ev_on(st)], 0)); // pool 2 starts after this point of st
+            for (int l = 1; l < npools; l++)
+                if (count) HIP_OK(hipMemsetAsync(lane_pool(h, l)->P.counters + kCntTraffic, 0, 32, st));
+            const int e = ev_on(st);
+            for (int l = 1; l < npools; l++) // the other pools start after this point of st
+                HIP_OK(hipStreamWaitEvent(lane_stream(h, l), evs[e], 0));
         }
         std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
         int first = 1, prev_acc = -1;
         for (int wave_base = 0, w = 0; wave_base < R.spp_eff; wave_base += k, w++) {
-            const bool second = two && (w & 1);
-            hipStream_t sw = second ? h->stream2 : st;
-            DPaths &P = second ? h->pb2.P : h->pb.P;
+            const int lane = w % npools;
+            hipStream_t sw = lane_stream(h, lane);
+            DPaths &P = lane_pool(h, lane)->P;
             auto ev = [&]() { return ev_on(sw); };
             int k_active = std::min(k, R.spp_eff - wave_base);
             int n = pixels * k_active;
@@ -577,12 +600,12 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             if (two) prev_acc = ev_on(sw);
             first = 0;
         }
-        if (two) HIP_OK(hipStreamWaitEvent(st, evs[prev_acc], 0)); // the last accumulate may be on stream2
+        if (two) HIP_OK(hipStreamWaitEvent(st, evs[prev_acc], 0)); // the last accumulate may be on another stream
         launch_finalize(R, h->pb.P, h->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
-        if (two) HIP_OK(hipStreamSynchronize(h->stream2));
+        if (two) sync_lanes(h);
         auto t1 = std::chrono::steady_clock::now();
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
@@ -602,13 +625,16 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             stats->paths = (uint64_t)pixels * (uint64_t)R.spp_eff;
             stats->extend_launches = ext_ev.size();
             if (count) {
-                uint64_t c[4], c2[4] = {0, 0, 0, 0};
-                HIP_OK(hipMemcpy(c, h->pb.P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
-                if (two) HIP_OK(hipMemcpy(c2, h->pb2.P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
-                stats->nodes_visited = c[0] + c2[0];
-                stats->tris_tested = c[1] + c2[1];
-                stats->rays_closest = c[2] + c2[2];
-                stats->rays_shadow = c[3] + c2[3];
+                uint64_t sum4[4] = {0, 0, 0, 0};
+                for (int l = 0; l < npools; l++) {
+                    uint64_t c[4];
+                    HIP_OK(hipMemcpy(c, lane_pool(h, l)->P.counters + kCntTraffic, 32, hipMemcpyDeviceToHost));
+                    for (int i = 0; i < 4; i++) sum4[i] += c[i];
+                }
+                stats->nodes_visited = sum4[0];
+                stats->tris_tested = sum4[1];
+                stats->rays_closest = sum4[2];
+                stats->rays_shadow = sum4[3];
             }
         }
         for (auto e : evs) (void)hipEventDestroy(e);
