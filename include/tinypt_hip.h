@@ -224,18 +224,19 @@ typedef struct {
     int32_t waves;          /* pipeline waves = replays of the captured hipGraph */
     int32_t pad_;
 } tpt_batch_stats;
-/* Renders n scenes (all created on the current device with the same node layout,
- * n <= 1024) through one pipeline: a pipeline wave (generate, then
- * max_depth + 1 x extend / shade / shadow) is captured once per path pool in a
- * hipGraph and replayed until all paths are done, alternating between two pools
- * on two streams (one with TPT_STREAMS=1); each wave's accumulation follows the
+/* Renders n scenes (all created on the current device with the same node layout, n
+ * <= 1024) through one pipeline: a pipeline wave (generate, then max_depth + 1 x
+ * extend / shade / shadow) is captured once per path pool in a hipGraph and
+ * replayed until all paths are done, alternating between up to three pools on as
+ * many streams (one with TPT_STREAMS=1); each wave's accumulation follows the
  * previous wave's, so the images are bit-identical to single-scene renders.
- * `params` gives the shared settings (max_depth, rr_depth, shadow_bounded,
- * output, accum_fp64); its spp and seed are ignored (per scene in `items`).  params->queue_capacity == 0 selects the
- * capacity by autotuning over 2^18..2^26 slots, params->block_size == 0 the
- * workgroup size over 64..512.  out_rgb[i] receives scene i's image (H*W*3
- * floats, row 0 at the bottom); `scene_ms` (may be NULL) receives n completion
- * times in ms since the batch started; `stats` may be NULL. */
+ * `params` gives the shared settings (max_depth, rr_depth, shadow_bounded, output,
+ * accum_fp64); its spp and seed are ignored (per scene in `items`).
+ * params->queue_capacity == 0 selects the capacity by autotuning over 2^18..2^26
+ * slots, params->block_size == 0 the workgroup size over 64..512.  out_rgb[i]
+ * receives scene i's image (H*W*3 floats, row 0 at the bottom); `scene_ms` (may be
+ * NULL) receives n completion times in ms since the batch started; `stats` may be
+ * NULL. */
 int tpt_hip_render_batch(tpt_hip_scene *const *scenes, const tpt_batch_item *items, int32_t n,
                          const tpt_render_params *params, float *const *out_rgb, tpt_batch_stats *stats,
                          double *scene_ms);

@@ -132,7 +132,7 @@ struct tpt_hip_scene {
     PathBuffers pbx[2];                 // path pools of frame lanes 2, 3 (cached renders), pb's accumulators
     int streams = 3;                    // plain renders alternate pipeline waves, cached and progressive
                                         // renders frames, over `streams` path pools on as many streams;
-                                        // batches use two pools from 2 on
+                                        // batches use up to three pools
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
     CacheState cache;
     ~tpt_hip_scene() {
@@ -491,8 +491,8 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
         // renders alternate pipeline waves between two streams (room 1024x768x256:
         // 542 vs 447 Mpaths/s, same image); TPT_STREAMS=1 keeps one
-        // lanes of plain renders and cached / progressive frames (batches: two pools
-        // from 2 on).  Room 1024x768x256: 668 / 699 / 656 Mpaths/s with 2 / 3 / 4;
+        // lanes of plain renders and cached / progressive frames (batches: up to
+        // three pools).  Room 1024x768x256: 668 / 699 / 656 Mpaths/s with 2 / 3 / 4;
         // textured room 1920x1080, 16 / 64 cached frames: 54.7 / 216.9 ms with 2,
         // 50.0 / 191.9 with 3, 57.2 / 220.7 with 4
         h->streams = env_int("TPT_STREAMS", 3, 1, 4);
@@ -653,10 +653,23 @@ namespace {
 // the scene's stream, pool 1 on a second one): one wave's shading overlaps the
 // other's traversal.  Both pools add into pool 0's per-pixel sums, each wave's
 // accumulation after the previous wave's, so the images do not change.
+// Pools of a batch at most (TPT_STREAMS permitting).  64 mixed rooms: 579 / 630
+// Mpaths/s with 2 / 3 pools; 256 small rooms: 510 / 519.
+#ifndef TPT_BATCH_POOLS
+#define TPT_BATCH_POOLS 3
+#endif
+constexpr int kBatchPools = TPT_BATCH_POOLS;
+
 struct BatchRun {
-    void *meta = nullptr, *meta2 = nullptr;
-    PathBuffers pb, pb2;
-    DBatch B{}, B2{};
+    static constexpr int kMaxPools = 4;
+    void *meta = nullptr;                 // scene / sampler tables, pool 0's segments and wave start
+    PathBuffers pb;                       // pool 0, holding the per-pixel sums
+    DBatch B{};                           // pool 0's view of the batch
+    hipStream_t st = nullptr;             // pool 0's stream
+    void *xmeta[kMaxPools - 1] = {};      // pools 1..: segment table and wave start
+    PathBuffers xpb[kMaxPools - 1];       // pools 1..: path state, adding into pool 0's sums
+    DBatch xB[kMaxPools - 1]{};
+    hipStream_t xst[kMaxPools - 1] = {};
     uint64_t total_paths = 0, total_pixels = 0;
     std::vector<uint64_t> path_off, pix_off;
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
@@ -665,83 +678,84 @@ struct BatchRun {
     int chunk = 384;       // queue entries per wave of the chunked kernels
     int max_lds = 65536;
     int pools = 1;
-    hipStream_t st = nullptr, st2 = nullptr;
+    DBatch &batch(int k) { return k ? xB[k - 1] : B; }
+    PathBuffers &pool(int k) { return k ? xpb[k - 1] : pb; }
+    hipStream_t stream(int k) const { return k ? xst[k - 1] : st; }
     ~BatchRun() {
-        if (st) (void)hipStreamSynchronize(st); // nothing queued may still use the buffers
-        if (st2) (void)hipStreamSynchronize(st2);
+        for (int k = 0; k < pools; k++) // nothing queued may still use the buffers
+            if (stream(k)) (void)hipStreamSynchronize(stream(k));
         if (pb.mem) (void)hipFree(pb.mem);
-        if (pb2.mem) (void)hipFree(pb2.mem);
         if (meta) (void)hipFree(meta);
-        if (meta2) (void)hipFree(meta2);
+        for (int k = 0; k < kMaxPools - 1; k++) {
+            if (xpb[k].mem) (void)hipFree(xpb[k].mem);
+            if (xmeta[k]) (void)hipFree(xmeta[k]);
+        }
     }
-    // the second pool: its own path state, queues, segment table and wave start
-    void add_pool(size_t slots, hipStream_t stream2) {
-        alloc_paths(pb2, slots, (size_t)total_pixels, true);
-        pb2.P.sub_f = pb.P.sub_f;
-        pb2.P.sub_d = pb.P.sub_d;
-        pb2.P.pix_f = pb.P.pix_f;
-        pb2.P.pix_d = pb.P.pix_d;
+    // one more pool: its own path state, queues, segment table and wave start
+    void add_pool(size_t slots, hipStream_t stream) {
+        const int k = pools;
+        PathBuffers &x = xpb[k - 1];
+        alloc_paths(x, slots, (size_t)total_pixels, true);
+        share_sums(x, pb);
         size_t seg_bytes = align256(4 * (size_t)kSegFields * (size_t)B.seg_stride);
-        HIP_OK(hipMalloc(&meta2, seg_bytes + 16));
-        HIP_OK(hipMemset(meta2, 0, seg_bytes + 16));
-        B2 = B;
-        B2.seg = (int32_t *)meta2;
-        B2.state = (uint64_t *)((uint8_t *)meta2 + seg_bytes);
-        B.pools = B2.pools = 2;
-        pools = 2;
-        st2 = stream2;
+        HIP_OK(hipMalloc(&xmeta[k - 1], seg_bytes + 16));
+        HIP_OK(hipMemset(xmeta[k - 1], 0, seg_bytes + 16));
+        DBatch &X = xB[k - 1];
+        X = B;
+        X.seg = (int32_t *)xmeta[k - 1];
+        X.state = (uint64_t *)((uint8_t *)xmeta[k - 1] + seg_bytes);
+        xst[k - 1] = stream;
+        pools = k + 1;
+        for (int q = 0; q < pools; q++) batch(q).pools = pools;
     }
     // capture the pipeline part of one wave of (cap, block), per pool, into executable graphs
-    void capture(int cap, int block, hipGraphExec_t ge[2]) {
-        B.cap = B2.cap = cap;
+    void capture(int cap, int block, hipGraphExec_t ge[kMaxPools]) {
+        for (int k = 0; k < pools; k++) batch(k).cap = cap;
         for (int k = 0; k < pools; k++) {
             hipGraph_t g = nullptr;
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_batch_wave(k ? B2 : B, k ? pb2.P : pb.P, max_depth, layout, block, trav_block, lds_stack, mode,
-                              chunk, max_lds, num_cu, total_pixels, st);
+            launch_batch_wave(batch(k), pool(k).P, max_depth, layout, block, trav_block, lds_stack, mode, chunk,
+                              max_lds, num_cu, total_pixels, st);
             HIP_OK(hipStreamEndCapture(st, &g));
             hipError_t e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             HIP_OK(e);
         }
     }
-    // restart at path `start` (a multiple of cap) with zeroed accumulators (the
-    // streams are drained first: the host copies are not ordered with the
-    // non-blocking streams)
+    // restart at path `start` with zeroed accumulators: pool k's first wave starts
+    // at start + k * cap (the streams are drained first: the host copies are not
+    // ordered with the non-blocking streams)
     void reset(int cap, uint64_t start = 0) {
-        HIP_OK(hipStreamSynchronize(st));
-        if (st2) HIP_OK(hipStreamSynchronize(st2));
-        uint64_t s0[2] = {start, total_paths};
-        HIP_OK(hipMemcpy(B.state, s0, sizeof(s0), hipMemcpyHostToDevice));
-        if (pools == 2) {
-            uint64_t s1[2] = {start + (uint64_t)cap, total_paths};
-            HIP_OK(hipMemcpy(B2.state, s1, sizeof(s1), hipMemcpyHostToDevice));
+        for (int k = 0; k < pools; k++) HIP_OK(hipStreamSynchronize(stream(k)));
+        for (int k = 0; k < pools; k++) {
+            uint64_t s0[2] = {start + (uint64_t)k * (uint64_t)cap, total_paths};
+            HIP_OK(hipMemcpy(batch(k).state, s0, sizeof(s0), hipMemcpyHostToDevice));
         }
         launch_batch_zero(B, pb.P, fp64, total_pixels, st);
     }
     int waves(int cap) const { return (int)((total_paths + (uint64_t)cap - 1) / (uint64_t)cap); }
-    // enqueue waves [0, W): pipeline graph on its pool's stream, then the tail after the
-    // previous wave's tail; done[w] (if given) is recorded after wave w
-    void run(hipGraphExec_t ge[2], int W, hipEvent_t *done) {
+    // enqueue waves [0, W): wave w's pipeline graph on pool w % pools's stream, then
+    // its tail after the previous wave's tail; done[w] (if given) is recorded after wave w
+    void run(hipGraphExec_t ge[kMaxPools], int W, hipEvent_t *done) {
         std::vector<hipEvent_t> tail(W);
         for (auto &e : tail) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        if (pools == 2) { // pool 1 starts after the reset on st
+        if (pools > 1) { // the other pools start after the reset on st
             hipEvent_t e0;
             HIP_OK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
             HIP_OK(hipEventRecord(e0, st));
-            HIP_OK(hipStreamWaitEvent(st2, e0, 0));
+            for (int k = 1; k < pools; k++) HIP_OK(hipStreamWaitEvent(stream(k), e0, 0));
             HIP_OK(hipEventDestroy(e0));
         }
         for (int w = 0; w < W; w++) {
-            const int k = pools == 2 ? (w & 1) : 0;
-            hipStream_t s = k ? st2 : st;
+            const int k = w % pools;
+            hipStream_t s = stream(k);
             HIP_OK(hipGraphLaunch(ge[k], s));
-            if (pools == 2 && w > 0) HIP_OK(hipStreamWaitEvent(s, tail[w - 1], 0));
-            launch_batch_tail(k ? B2 : B, k ? pb2.P : pb.P, total_pixels, s);
+            if (pools > 1 && w > 0) HIP_OK(hipStreamWaitEvent(s, tail[w - 1], 0));
+            launch_batch_tail(batch(k), pool(k).P, total_pixels, s);
             HIP_OK(hipEventRecord(tail[w], s));
             if (done) HIP_OK(hipEventRecord(done[w], s));
         }
-        if (pools == 2 && W > 0) HIP_OK(hipStreamWaitEvent(st, tail[W - 1], 0)); // st continues after the last wave
+        if (pools > 1 && W > 0) HIP_OK(hipStreamWaitEvent(st, tail[W - 1], 0)); // st continues after the last wave
         for (auto &e : tail) HIP_OK(hipEventDestroy(e));
     }
 };
@@ -751,7 +765,7 @@ struct BatchRun {
 // capacity, or reaches the batch's end), so the comparison does not depend on
 // which scenes a candidate's waves happen to cover.
 double tune_trial(BatchRun &run, int cap, int block, uint64_t start, uint64_t span) {
-    hipGraphExec_t ge[2] = {nullptr, nullptr};
+    hipGraphExec_t ge[BatchRun::kMaxPools] = {};
     run.capture(cap, block, ge);
     const uint64_t end = std::min(run.total_paths, start + span);
     const int reps = (int)((end - start + (uint64_t)cap - 1) / (uint64_t)cap);
@@ -893,10 +907,11 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         if (p->block_size > 0) blocks.push_back(p->block_size);
         else blocks = {64, 128, 256, 512};
         alloc_paths(run.pb, (size_t)*std::max_element(caps.begin(), caps.end()), (size_t)run.total_pixels, true);
-        if (scenes[0]->streams >= 2 && run.total_paths > (uint64_t)caps[0]) { // two waves in flight
-            if (!scenes[0]->stream2) HIP_OK(hipStreamCreateWithFlags(&scenes[0]->stream2, hipStreamNonBlocking));
-            run.add_pool((size_t)*std::max_element(caps.begin(), caps.end()), scenes[0]->stream2);
-        }
+        // several waves in flight: one pool per lane of the first scene
+        const int npools = run.total_paths > (uint64_t)caps[0] ? std::min(scenes[0]->streams, kBatchPools) : 1;
+        ensure_lane_streams(scenes[0], npools);
+        for (int k = 1; k < npools; k++)
+            run.add_pool((size_t)*std::max_element(caps.begin(), caps.end()), lane_stream(scenes[0], k));
         int best_cap = caps[0], best_block = blocks.size() == 1 ? blocks[0] : 256;
         double tune_ms = 0;
         if (caps.size() > 1 || blocks.size() > 1) {
@@ -928,7 +943,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         }
         // the render: one captured pipeline wave per pool, replayed until every path is done
         auto tr = std::chrono::steady_clock::now();
-        hipGraphExec_t ge[2] = {nullptr, nullptr};
+        hipGraphExec_t ge[BatchRun::kMaxPools] = {};
         run.capture(best_cap, best_block, ge);
         run.reset(best_cap);
         int W = run.waves(best_cap);
