@@ -11,9 +11,10 @@ of the extend launches) and the CPU oracle timed on a bounded sample.
 
 Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
-                      16-256 spp) through PathTracer.render_batch: two path
-                      pools on two streams, a captured pipeline wave per pool
-                      replayed, autotuned queue capacity and workgroup size;
+                      16-256 spp) through PathTracer.render_batch: up to three
+                      path pools on as many streams, a captured pipeline wave
+                      per pool replayed, autotuned queue capacity and
+                      workgroup size;
                       per-scene completion times and the same scenes rendered
                       one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
@@ -45,7 +46,7 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
 # FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
 # read inside this process, so the figure is the profiled one; it is reported only for that workload.
-EXTEND_TRAFFIC_KIB = {"fetch": 1227807.5, "fetch_scale": 2.0, "write": 260623.9, "write_scale": 1.0}
+EXTEND_TRAFFIC_KIB = {"fetch": 1234153.1, "fetch_scale": 2.0, "write": 258294.1, "write_scale": 1.0}
 DEFAULT_ROOM = (1024, 768, 256)
 
 
@@ -310,7 +311,7 @@ def bench_batch(args):
     out = base_line(paths / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
         "workload": (f"batch_{len(devs)}_rooms_{ranges['tris'][0] // 1000}k-{ranges['tris'][1] // 1000}k_tris_"
                      f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8_rr5"),
-        "tracer": "plain HIP wavefront, two path pools on two streams, captured pipeline wave per pool replayed",
+        "tracer": "plain HIP wavefront, three path pools on three streams, captured pipeline wave per pool replayed",
         "paths_per_step": paths,
         "queue_capacity": cap,
         "block_size": block,
