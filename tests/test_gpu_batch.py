@@ -1,8 +1,9 @@
-"""Batch mode: many independent scenes through one shared path pool and one
-captured pipeline wave.  Paths run scene-major and sample-major, so every
-pixel's samples are accumulated in sample order and each scene's image must
-equal its single-scene render bit for bit, whatever the queue capacity (pipeline
-waves that split a scene, or hold several scenes) and the workgroup size."""
+"""Batch mode: many independent scenes through one captured pipeline wave,
+replayed on up to three path pools (one per stream of the first scene).  Paths
+run scene-major and sample-major, so every pixel's samples are accumulated in
+sample order and each scene's image must equal its single-scene render bit for
+bit, whatever the queue capacity (pipeline waves that split a scene, or hold
+several scenes), the workgroup size and the number of pools."""
 import numpy as np
 import pytest
 
@@ -70,3 +71,17 @@ def test_batch_rejects_bad_input(hip, batch):
         pt.render_batch(devs[:1], [0], [1])
     with pytest.raises(RuntimeError, match="Expect scene on hip"):
         pt.render_batch([hosts[0]], [4], [1])
+
+
+@pytest.mark.parametrize("streams", ["1", "2"])
+def test_batch_pools_equal_single_renders(hip, batch, monkeypatch, streams):
+    # the batch takes its pool count from the first scene's streams (TPT_STREAMS at
+    # scene creation): one and two pools give the single renders' images too
+    hosts, _, spps, seeds = batch
+    monkeypatch.setenv("TPT_STREAMS", streams)
+    devs = [h.to("hip") for h in hosts]
+    kw = dict(max_depth=6, rr_depth=3, hdr=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    imgs = hip.PathTracer("hip").render_batch(devs, spps, seeds, queue_capacity=4096, block_size=64, **kw)
+    for i, (a, b) in enumerate(zip(imgs, ref)):
+        assert np.array_equal(a.raw(), b), f"scene {i} differs"
