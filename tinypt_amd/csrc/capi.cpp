@@ -191,6 +191,8 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     size_t o_li = put(blob, li);
     size_t o_em = put(blob, H.env_marg);
     size_t o_ec = put(blob, H.env_cond);
+    size_t o_gm = put(blob, H.env_gmarg);
+    size_t o_gc = put(blob, H.env_gcond);
     size_t total = align256(blob.size());
     blob.resize(total);
     HIP_OK(hipMalloc(&h->arena, total));
@@ -214,6 +216,8 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.lights = (const DLight *)(base + o_li);
     S.env_marg = (const float *)(base + o_em);
     S.env_cond = (const float *)(base + o_ec);
+    S.env_gmarg = (const int32_t *)(base + o_gm);
+    S.env_gcond = (const int32_t *)(base + o_gc);
     S.layout = H.layout;
     const char *pers = std::getenv("TPT_PERSISTENT");
     // traversal kernels (room 1024x768x256, Mpaths/s):
@@ -248,6 +252,8 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.num_lights = (int32_t)H.lights.size();
     S.env_w = H.env_w;
     S.env_h = H.env_h;
+    S.env_gw = H.env_gw;
+    S.env_gh = H.env_gh;
     S.bg_map = H.bg_map;
     S.bg = H.bg;
     std::memcpy(S.bg_rot, H.bg_rot, sizeof(S.bg_rot));
@@ -853,7 +859,8 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             to_offset(T.nodes), to_offset(T.tris_bvh), to_offset(T.tris), to_offset(T.tri_info), to_offset(T.shade_rec);
             to_offset(T.normals), to_offset(T.texcoords), to_offset(T.gid_local), to_offset(T.analytic);
             to_offset(T.materials), to_offset(T.images), to_offset(T.texels), to_offset(T.delta);
-            to_offset(T.lights), to_offset(T.env_marg), to_offset(T.env_cond);
+            to_offset(T.lights), to_offset(T.env_marg), to_offset(T.env_cond), to_offset(T.env_gmarg);
+            to_offset(T.env_gcond);
             table[i].lds_nodes = 0; // no LDS node cache: wavefronts of one workgroup trace different scenes
             int px = table[i].cam.width * table[i].cam.height;
             rend[i] = make_render(p, items[i].spp, items[i].seed, px);

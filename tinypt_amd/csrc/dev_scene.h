@@ -77,9 +77,16 @@ __device__ __forceinline__ V3 background(const DScene &S, V3 dir) { // EnvTextur
     return color_at(S, S.bg, uv);
 }
 
-// piecewise-constant CDF (cpu/distribution.cpp:32-44); cdf has count+1 entries
-__device__ __forceinline__ float dist_sample(const float *cdf, int count, float u) {
-    int lo = 0, hi = count + 1; // upper_bound over [0, count]
+// piecewise-constant CDF (cpu/distribution.cpp:32-44); cdf has count+1 entries.
+// `guide` (G + 1 entries, G a power of two) narrows the upper_bound over
+// [0, count] to [guide[k], guide[k + 1]] for k = floor(u * G): every index below
+// guide[k] has cdf <= k / G <= u, and cdf[guide[k + 1]] > (k + 1) / G > u, so the
+// narrowed search returns the same index in one or two steps instead of
+// log2(count) dependent loads.
+__device__ __forceinline__ float dist_sample(const float *cdf, int count, const int32_t *guide, int G, float u) {
+    int k = (int)(u * (float)G); // exact: G is a power of two, u in [0, 1]
+    k = k < G ? k : G - 1;
+    int lo = guide[k], hi = guide[k + 1];
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
         if (u < cdf[mid]) hi = mid; else lo = mid + 1;
@@ -95,9 +102,10 @@ __device__ __forceinline__ float dist_pdf(const float *cdf, int count, float x) 
 __device__ __forceinline__ V3 env_sample(const DScene &S, Rng &rng) { // texture.cpp:79-84
     float u = rng.random();
     float v = rng.random();
-    float y = dist_sample(S.env_marg, S.env_h, v);
+    float y = dist_sample(S.env_marg, S.env_h, S.env_gmarg, S.env_gh, v);
     int yi = dist_anchor(S.env_h, y);
-    float x = dist_sample(S.env_cond + (size_t)yi * (S.env_w + 1), S.env_w, u);
+    float x = dist_sample(S.env_cond + (size_t)yi * (S.env_w + 1), S.env_w, S.env_gcond + (size_t)yi * (S.env_gw + 1),
+                          S.env_gw, u);
     return mat_apply_t(S.bg_rot, uv2dir(V2{x, y}));
 }
 __device__ __forceinline__ float env_pdf(const DScene &S, V3 dir) { // texture.cpp:86-90

@@ -141,6 +141,10 @@ struct DScene {
     const DLight *lights;
     const float *env_marg; // env_h + 1 floats
     const float *env_cond; // env_h * (env_w + 1) floats
+    // guide tables of the CDFs (env_gh + 1 and env_h * (env_gw + 1) entries): entry k of
+    // a table is upper_bound(cdf, k / G) for k < G, and count + 1 for k = G
+    const int32_t *env_gmarg;
+    const int32_t *env_gcond;
     int32_t num_nodes;
     int32_t root; // child reference of the root (inner 0, or a leaf when the BVH is one leaf), -1 if empty
     int32_t num_tris;
@@ -160,7 +164,9 @@ struct DScene {
     int32_t trav_block; // threads per workgroup of the persistent traversal kernels
     int32_t max_lds;    // bytes of LDS one workgroup may allocate on the device
     int32_t chunk;      // queue entries per wave of the chunked traversal kernels (persistent 3, 4)
-    int32_t pad4;
+    int32_t env_gh;     // guide buckets of the marginal CDF (a power of two)
+    int32_t env_gw;     // guide buckets of each conditional CDF (a power of two)
+    int32_t pad5;
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

@@ -1301,6 +1301,8 @@ __device__ __forceinline__ DScene scene_at(const DBatch &B, int s) {
     S.lights = rebase(B.base, S.lights);
     S.env_marg = rebase(B.base, S.env_marg);
     S.env_cond = rebase(B.base, S.env_cond);
+    S.env_gmarg = rebase(B.base, S.env_gmarg);
+    S.env_gcond = rebase(B.base, S.env_gcond);
     return S;
 }
 

@@ -728,6 +728,28 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
         H.env_cond = {0.f, 1.f};
         H.env_marg = {0.f, 1.f};
     }
+    // guide tables: sampling a CDF of `count` bins with u in bucket k = floor(u * G)
+    // needs upper_bound only over [g[k], g[k + 1]] (one or two entries), the same
+    // index as the search over the whole CDF
+    auto pow2_at_least = [](int n) {
+        int g = 1;
+        while (g < n) g <<= 1;
+        return g;
+    };
+    auto build_guide = [](const float *cdf, int count, int G, std::vector<int32_t> &out) {
+        for (int k = 0; k < G; k++) {
+            const float t = (float)k / (float)G; // exact: G is a power of two
+            out.push_back((int32_t)(std::upper_bound(cdf, cdf + count + 1, t) - cdf));
+        }
+        out.push_back(count + 1);
+    };
+    H.env_gh = pow2_at_least(H.env_h);
+    H.env_gw = pow2_at_least(H.env_w);
+    H.env_gmarg.clear();
+    H.env_gcond.clear();
+    build_guide(H.env_marg.data(), H.env_h, H.env_gh, H.env_gmarg);
+    for (int y = 0; y < H.env_h; y++)
+        build_guide(H.env_cond.data() + (size_t)y * (H.env_w + 1), H.env_w, H.env_gw, H.env_gcond);
     if (H.bg_map) H.lights.push_back(DLight{L_ENV, -1, -1, 0});
 
     // ---- camera (cpu/camera.cpp:6-21)
