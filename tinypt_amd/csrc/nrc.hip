@@ -104,39 +104,102 @@ __device__ __forceinline__ void load_tile(uint16_t *dst, const uint16_t *feats, 
 }
 
 // out[i]: predicted radiance (rgb, 0) of record i.
-__global__ void k_nrc_infer(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
-                            const uint32_t *__restrict__ n_dev, int n_max, const uint16_t *__restrict__ wt,
-                            float4 *__restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint16_t act[2][4][64 * kRow];
+//
+// The activations stay in registers: each layer computes y^T = W^T x^T with A =
+// wt[l] (rows = outputs, 8 K values per lane) and B = x^T (8 K values of one
+// sample per lane).  The MFMA result D holds, per lane, 4 consecutive outputs
+// (g * 4 + j within a 16-output tile, g = lane / 16) of sample lane % 16, which
+// after ReLU and BF16 rounding is the next layer's B operand if the K order of
+// step s is permuted: lane group g holds features s * 32 + g * 4 + {0..3} (from
+// output tile 2s) and s * 32 + 16 + g * 4 + {0..3} (tile 2s + 1).  The A operand
+// reads the same features of a wt row (two 8-byte runs).  The first layer's B
+// comes straight from the feature rows in memory (identity K order).  No LDS and
+// no barriers: a wavefront owns its 64-record tile from load to store.
+__device__ __forceinline__ bf16x8 ld4x2(const uint16_t *p0, const uint16_t *p1) {
+    uint2 a = *reinterpret_cast<const uint2 *>(p0), b = *reinterpret_cast<const uint2 *>(p1);
+    uint4 v = make_uint4(a.x, a.y, b.x, b.y);
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, 16);
+    return r;
+}
+__device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
+    uint16_t v[8];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        v[j] = f2bf(fmaxf(lo[j], 0.f));
+        v[4 + j] = f2bf(fmaxf(hi[j], 0.f));
+    }
+    bf16x8 r;
+    __builtin_memcpy(&r, v, 16);
+    return r;
+}
+
+// NT sample tiles of 16 per wavefront (tile of NT * 16 records).
+template <int NT>
+__global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
+                                                   const uint32_t *__restrict__ n_dev, int n_max,
+                                                   const uint16_t *__restrict__ wt, float4 *__restrict__ out) {
     const int n = rec_count(n_dev, n_max);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int tile0 = (blockIdx.x * (blockDim.x >> 6) + w) * 64;
-    if (blockIdx.x * (blockDim.x >> 6) * 64 >= n) return; // whole workgroup beyond the records
-    uint16_t *a0 = act[0][w], *a1 = act[1][w];
-    load_tile(a0, feats, idx, tile0, n);
-    __syncthreads();
-    float pred[4][4];
-    layer_fwd<false>(a0, a1, wt + 0 * 4096, pred);
-    __syncthreads();
-    layer_fwd<false>(a1, a0, wt + 1 * 4096, pred);
-    __syncthreads();
-    layer_fwd<false>(a0, a1, wt + 2 * 4096, pred);
-    __syncthreads();
-    layer_fwd<false>(a1, a0, wt + 3 * 4096, pred);
-    __syncthreads();
-    layer_fwd<true>(a0, a1, wt + 4 * 4096, pred);
-    // lanes with output column r < 3 hold the predictions; gather them through LDS
-    float *o = reinterpret_cast<float *>(a1); // 64 x 4 floats (reuses a1)
-    __syncthreads();
-    const int r = l & 15, q = l >> 4;
-    if (r < 3)
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int tile0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (NT * 16);
+    if (tile0 >= n) return; // whole wavefront beyond the records (no barriers below)
+    // layer 0 operand: features s * 32 + g * 8 + {0..7} of sample nt * 16 + r
+    bf16x8 xb[NT][2];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) {
+        const int i = tile0 + nt * 16 + r;
+        const uint16_t *row = rec_row(feats, idx, i < n ? i : tile0);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            xb[nt][s] = ld8(row + s * 32 + g * 8);
+            if (i >= n) xb[nt][s] = bf16x8{};
+        }
+    }
+#pragma unroll
+    for (int ly = 0; ly < 4; ly++) {
+        const uint16_t *W = wt + ly * 4096;
+        f32x4 acc[4][NT];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) o[(mt * 16 + q * 4 + j) * 4 + r] = pred[mt][j];
-    __syncthreads();
-    if (tile0 + l < n) out[tile0 + l] = make_float4(o[l * 4], o[l * 4 + 1], o[l * 4 + 2], 0.f);
+            for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const uint16_t *row = W + (mt * 16 + r) * 64 + s * 32;
+                const bf16x8 a = ly == 0 ? ld8(row + g * 8) : ld4x2(row + g * 4, row + 16 + g * 4);
+#pragma unroll
+                for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma(a, xb[nt][s], acc[mt][nt]);
+            }
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+            for (int s = 0; s < 2; s++) xb[nt][s] = relu_pack(acc[2 * s][nt], acc[2 * s + 1][nt]);
+    }
+    // output layer: one 16-output tile (outputs 0..2 are the rgb prediction)
+    const uint16_t *W = wt + 4 * 4096;
+    f32x4 o[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const uint16_t *row = W + r * 64 + s * 32;
+        const bf16x8 a = ld4x2(row + g * 4, row + 16 + g * 4);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) o[nt] = mfma(a, xb[nt][s], o[nt]);
+    }
+    if (g == 0) // lanes 0..15 hold outputs 0..3 of sample nt * 16 + r
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const int i = tile0 + nt * 16 + r;
+            if (i < n) out[i] = make_float4(o[nt][0], o[nt][1], o[nt][2], 0.f);
+        }
 }
+#ifndef TPT_NRC_INFER_TILES
+#define TPT_NRC_INFER_TILES 2
+#endif
+static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per wavefront
 
 // ---------------------------------------------------------------- training
 // Forward + backward over one 64-sample tile per wavefront (one wavefront per
@@ -278,9 +341,10 @@ __global__ void k_nrc_adam(float *__restrict__ w, float *__restrict__ m, float *
 void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t *n_dev, int n_max,
                       const uint16_t *wt, float4 *out, hipStream_t st) {
     if (n_max <= 0) return;
-    int tiles = (n_max + 63) / 64;
-    int blocks = (tiles + 3) / 4;
-    hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
+    const int per_wave = kInferTiles * 16;
+    int waves = (n_max + per_wave - 1) / per_wave;
+    int blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(k_nrc_infer<kInferTiles>, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
 }
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
                       int base, int n_max, const NrcWeights &W, hipStream_t st) {
