@@ -1122,11 +1122,9 @@ struct FrameChain {
     FrameChain() = default;
     FrameChain(const FrameChain &) = delete;
     FrameChain &operator=(const FrameChain &) = delete;
-    void create() {
-        if (acc) return;
-        HIP_OK(hipEventCreateWithFlags(&train, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&acc, hipEventDisableTiming));
-        HIP_OK(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+    void create() { // each missing event (a failed call leaves the others to the destructor)
+        for (hipEvent_t *e : {&train, &acc, &start})
+            if (!*e) HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     ~FrameChain() {
         for (hipEvent_t e : {train, acc, start})
