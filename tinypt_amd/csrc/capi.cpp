@@ -359,6 +359,17 @@ void ensure_stream2(tpt_hip_scene *h) {
     if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
 }
 
+// HIP events owned by a render: destroyed on every exit, errors included.
+struct EventVec : std::vector<hipEvent_t> {
+    explicit EventVec(size_t n = 0) : std::vector<hipEvent_t>(n, nullptr) {}
+    EventVec(const EventVec &) = delete;
+    EventVec &operator=(const EventVec &) = delete;
+    ~EventVec() {
+        for (hipEvent_t e : *this)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // Frame lanes (cached and progressive renders): lane l runs on its own stream.
 constexpr int kMaxLanes = 4;
 hipStream_t lane_stream(const tpt_hip_scene *h, int l) {
@@ -557,12 +568,12 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         bool timing = stats != nullptr;
         bool count = p->count_traffic != 0;
         if (count) HIP_OK(hipMemsetAsync(h->pb.P.counters + kCntTraffic, 0, 32, st));
-        std::vector<hipEvent_t> evs;
+        EventVec evs;
         auto ev_on = [&](hipStream_t s) {
             hipEvent_t e;
             HIP_OK(hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming));
-            HIP_OK(hipEventRecord(e, s));
             evs.push_back(e);
+            HIP_OK(hipEventRecord(e, s));
             return (int)evs.size() - 1;
         };
         if (two) {
@@ -643,7 +654,6 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
                 stats->rays_shadow = sum4[3];
             }
         }
-        for (auto e : evs) (void)hipEventDestroy(e);
         return TPT_OK;
     } catch (const tpt_error &e) {
         return set_err(e.code, e.what());
@@ -954,7 +964,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         run.capture(best_cap, best_block, ge);
         run.reset(best_cap);
         int W = run.waves(best_cap);
-        std::vector<hipEvent_t> ev(W + 1);
+        EventVec ev(W + 1);
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
         HIP_OK(hipEventRecord(ev[0], run.st));
         run.run(ge, W, ev.data() + 1);
@@ -973,7 +983,6 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
                 scene_ms[order[i]] = ms;
             }
         }
-        for (auto e : ev) (void)hipEventDestroy(e);
         for (int k = 0; k < run.pools; k++) (void)hipGraphExecDestroy(ge[k]);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
@@ -1280,7 +1289,7 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         hipStream_t st = h->stream;
         CacheState &c = h->cache;
         // four events per frame (inference and training brackets), read after the last frame
-        std::vector<hipEvent_t> ev(fs.use_cache ? 4 * (size_t)cp->frames : 0);
+        EventVec ev(fs.use_cache ? 4 * (size_t)cp->frames : 0);
         for (auto &x : ev) HIP_OK(hipEventCreate(&x));
         FrameChain ch;
         enqueue_frames(h, fs, pools, ch, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
@@ -1304,7 +1313,6 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
             infer_ms += a;
             train_ms += b2;
         }
-        for (auto &x : ev) (void)hipEventDestroy(x);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
