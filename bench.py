@@ -46,7 +46,7 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
 # FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
 # read inside this process, so the figure is the profiled one; it is reported only for that workload.
-EXTEND_TRAFFIC_KIB = {"fetch": 1234153.1, "fetch_scale": 2.0, "write": 258294.1, "write_scale": 1.0}
+EXTEND_TRAFFIC_KIB = {"fetch": 1236983.0, "fetch_scale": 2.0, "write": 258748.9, "write_scale": 1.0}
 DEFAULT_ROOM = (1024, 768, 256)
 
 
@@ -216,8 +216,8 @@ def main():
             "node_bytes": info["node_bytes"],
             "bytes_per_launch": alg_bytes / max(1, c["extend_launches"]),
             "avg_launch_ms": (sum(ext_ms) / len(ext_ms)) / max(1, c["extend_launches"]),
-            # the timed renders run two pipeline waves at once on two streams, so each
-            # extend launch shares the CUs with the other stream's shading; alone:
+            # the timed renders run pipeline waves at once on several streams (three by
+            # default), so each extend launch shares the CUs with the others' shading; alone:
             "isolated": {"streams": 1, "achieved": round(iso_achieved, 1),
                          "frac": round(iso_achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": iso_ms},
         }

@@ -231,7 +231,11 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     // the queue and serialises every refill at one L2 channel (L2 read latency 700 vs 233 cycles).
     S.persistent = 4;
     if (pers && std::string(pers).size() == 1 && pers[0] >= '0' && pers[0] <= '4') S.persistent = pers[0] - '0';
-    S.chunk = env_int("TPT_CHUNK", 384, 64, 1 << 20) / 64 * 64;
+    // room 1024x768x256, three streams: chunks of 384 / 768 / 1024 give 703-707 /
+    // 707-720 / 713 Mpaths/s, and the co-scheduled extend launches run at 0.66 /
+    // 0.70-0.75 / 0.71 of the roofline (larger chunks leave more CUs to the other
+    // streams' kernels)
+    S.chunk = env_int("TPT_CHUNK", 768, 64, 1 << 20) / 64 * 64;
     // traversal stack: the first kLdsStack entries in LDS by default (the private
     // scratch stack spills to memory: ~9x the extend kernel's HBM writes);
     // TPT_LDS_STACK=0 selects the private stack
@@ -691,7 +695,7 @@ struct BatchRun {
     int max_depth = 0, layout = 0, trav_block = 512, num_cu = 256, fp64 = 0;
     bool lds_stack = true; // first traversal-stack entries of each lane in LDS
     int mode = 4;          // traversal kernels (DScene::persistent)
-    int chunk = 384;       // queue entries per wave of the chunked kernels
+    int chunk = 768;       // queue entries per wave of the chunked kernels
     int max_lds = 65536;
     int pools = 1;
     DBatch &batch(int k) { return k ? xB[k - 1] : B; }
