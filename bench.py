@@ -188,7 +188,7 @@ def main():
         if (args.width, args.height, args.spp) == DEFAULT_ROOM:
             t = EXTEND_TRAFFIC_KIB
             traffic = round((t["fetch"] * t["fetch_scale"] + t["write"] * t["write_scale"]) * 1024)
-        # the same kernel without the second stream's co-running waves (a scene handle
+        # the same kernel without the other streams' co-running waves (a scene handle
         # created with TPT_STREAMS=1, one untimed warm-up render, then one measured)
         prev = os.environ.get("TPT_STREAMS")
         os.environ["TPT_STREAMS"] = "1"

@@ -217,7 +217,7 @@ def test_several_streams_render_identically(hip, monkeypatch, hdr, streams):
     for streams in ("1", streams):
         monkeypatch.setenv("TPT_STREAMS", streams)
         dev = host.to("hip")
-        # 7 waves of 10 samples per pixel (the last one partial): both streams take several
+        # 7 waves of 10 samples per pixel (the last one partial): every stream takes one or more
         images.append(hip.PathTracer("hip").render(dev, 64, seed=8, max_depth=8, rr_depth=5, hdr=hdr,
                                                    queue_capacity=96 * 64 * 10).raw())
     assert np.array_equal(images[0], images[1])

@@ -123,13 +123,13 @@ struct tpt_hip_scene {
     DScene S{};
     uint64_t num_nodes = 0, num_tris = 0, num_analytic = 0;
     PathBuffers pb;
-    PathBuffers pb2;                    // second path pool (two-stream renders), pb's accumulators
+    PathBuffers pb2;                    // path pool of lane 1 (several-stream renders), pb's accumulators
     uint32_t *trace_counters = nullptr; // 8 x u32
     uint64_t last_nodes = 0, last_tris = 0;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;      // second stream (two-stream renders), default priority
-    hipStream_t streamx[2] = {nullptr, nullptr}; // third and fourth streams (frame lanes 2, 3)
-    PathBuffers pbx[2];                 // path pools of frame lanes 2, 3 (cached renders), pb's accumulators
+    hipStream_t stream2 = nullptr;      // stream of lane 1 (several-stream renders), default priority
+    hipStream_t streamx[2] = {nullptr, nullptr}; // streams of lanes 2, 3
+    PathBuffers pbx[2];                 // path pools of lanes 2, 3, pb's accumulators
     int streams = 3;                    // plain renders alternate pipeline waves, cached and progressive
                                         // renders frames, over `streams` path pools on as many streams;
                                         // batches use up to three pools
@@ -510,10 +510,10 @@ int tpt_hip_scene_create(const tpt_scene_desc *desc, tpt_hip_scene **out) {
         }
         h->S.lds_nodes = std::min(h->S.lds_nodes, (max_shared - stack_bytes) / 64);
         h->S.max_lds = max_shared; // one-ray-per-lane launches fit their LDS per block size
-        // renders alternate pipeline waves between two streams (room 1024x768x256:
-        // 542 vs 447 Mpaths/s, same image); TPT_STREAMS=1 keeps one
-        // lanes of plain renders and cached / progressive frames (batches: up to
-        // three pools).  Room 1024x768x256: 668 / 699 / 656 Mpaths/s with 2 / 3 / 4;
+        // lanes (a path pool on its own stream each): plain renders rotate pipeline
+        // waves, cached / progressive renders frames, over them, with the same image
+        // as one lane (batches: up to three pools); TPT_STREAMS=1 keeps one.  Two
+        // lanes against one took the room from 447 to 542 Mpaths/s when added.  Room 1024x768x256: 668 / 699 / 656 Mpaths/s with 2 / 3 / 4;
         // textured room 1920x1080, 16 / 64 cached frames: 54.7 / 216.9 ms with 2,
         // 50.0 / 191.9 with 3, 57.2 / 220.7 with 4
         h->streams = env_int("TPT_STREAMS", 3, 1, 4);
@@ -669,10 +669,10 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
 namespace {
 
 // Device state of one batch render: scene / sampler tables, segment tables and
-// path pools.  With two pools, pipeline waves alternate between them (pool 0 on
-// the scene's stream, pool 1 on a second one): one wave's shading overlaps the
-// other's traversal.  Both pools add into pool 0's per-pixel sums, each wave's
-// accumulation after the previous wave's, so the images do not change.
+// path pools.  With several pools, pipeline waves rotate over them (pool 0 on
+// the scene's stream, the others on streams of their own): one wave's shading
+// overlaps the others' traversal.  All pools add into pool 0's per-pixel sums,
+// each wave's accumulation after the previous wave's, so the images do not change.
 // Pools of a batch at most (TPT_STREAMS permitting).  64 mixed rooms: 579 / 630
 // Mpaths/s with 2 / 3 pools; 256 small rooms: 510 / 519.
 #ifndef TPT_BATCH_POOLS
@@ -1128,7 +1128,7 @@ constexpr int kFrameBlock = 256;
 // Orders frames that run on different streams: `train` is recorded after the
 // latest frame's training (the next inference reads the weights it wrote),
 // `acc` after its accumulation (the per-pixel sums are shared and added in
-// frame order), `start` where the second stream joins the first.
+// frame order), `start` where the other lanes' streams join the scene's stream.
 struct FrameChain {
     hipEvent_t train = nullptr, acc = nullptr, start = nullptr;
     bool any = false; // a frame has been enqueued: later frames wait on the events
@@ -1297,7 +1297,7 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         for (auto &x : ev) HIP_OK(hipEventCreate(&x));
         FrameChain ch;
         enqueue_frames(h, fs, pools, ch, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
-        HIP_OK(hipStreamWaitEvent(st, ch.acc, 0)); // the last frame may have run on the second stream
+        HIP_OK(hipStreamWaitEvent(st, ch.acc, 0)); // the last frame may have run on another lane's stream
         launch_frame_finalize(h->pb.P, pixels, fs.fp64, cp->frames, h->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, h->pb.out, (size_t)pixels * 12, hipMemcpyDeviceToHost, st));
@@ -1418,7 +1418,7 @@ int tpt_hip_progress_read(tpt_hip_progress *s, float *out_rgb, int32_t *frames) 
     try {
         HIP_OK(hipSetDevice(s->scene->device));
         hipStream_t st = s->scene->stream;
-        HIP_OK(hipStreamWaitEvent(st, s->chain.acc, 0)); // the last frame may have run on the second stream
+        HIP_OK(hipStreamWaitEvent(st, s->chain.acc, 0)); // the last frame may have run on another lane's stream
         launch_frame_finalize(s->pb.P, s->fs.pixels, s->fs.fp64, s->frames, s->pb.out, st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(out_rgb, s->pb.out, (size_t)s->fs.pixels * 12, hipMemcpyDeviceToHost, st));
