@@ -375,6 +375,7 @@ def bench_cache(args):
         cached.append(dict(frames=frames, ms=round(dt * 1e3, 1), relmse=round(e, 5),
                            plain_ms_same_relmse=round(c / e * 1e3, 1), speedup=round(c / e / dt, 3),
                            infer_ms=round(st["infer_ms"], 1), train_ms=round(st["train_ms"], 1),
+                           queries=int(st["queries"]), records=int(st["records"]), steps=int(st["steps"]),
                            mean_bias=round(float(img.mean() / ref.mean() - 1.0), 4)))
         print(f"# cached {frames:4d} frames: {dt * 1e3:8.1f} ms relMSE {e:.5f} -> equal-error speedup "
               f"{cached[-1]['speedup']:.2f} (bias {cached[-1]['mean_bias']:+.4f})", file=sys.stderr, flush=True)
