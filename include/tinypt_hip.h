@@ -303,7 +303,10 @@ int tpt_hip_cache_train(tpt_hip_scene *scene, const uint16_t *inputs, const floa
  * the frames on the scene's stream and returns; wait and read block until they
  * are done; read writes the HDR mean of the frames so far (row 0 at the bottom).
  * The session owns its path pool; with a cache it trains the scene handle's
- * network.  Calls on one scene handle are ordered on its stream. */
+ * network.  Frames run on the scene handle's streams (frame f on lane
+ * f % TPT_STREAMS); the training steps of every session and cached render of one
+ * scene handle run one after another, and the cache hooks (weights, eval, train)
+ * wait for all of the handle's streams first. */
 typedef struct tpt_hip_progress tpt_hip_progress;
 int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *params, const tpt_cache_params *cache,
                             tpt_hip_progress **out);

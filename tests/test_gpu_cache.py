@@ -19,14 +19,30 @@ def bits(x):
     return (bf16(x).view(np.uint32) >> 16).astype(np.uint16)
 
 
+def mm(a, b):
+    """a @ b for BF16-valued a, b as the MFMA sums them: the products of each group
+    of 8 consecutive K values are summed exactly (BF16 products are exact in
+    FP32), rounded to FP32 and added to the FP32 accumulator in K order.  Another
+    FP32 order (BLAS's) moves a pre-activation that sits within rounding of a
+    BF16 rounding boundary or of zero; the BF16-rounded activations and the ReLU
+    mask then differ and the gradient differences grow layer by layer (measured
+    up to 9e-3 in the first layer's gradient on such inputs), while this order
+    reproduces the kernels to ~1e-7."""
+    acc = np.zeros((a.shape[0], b.shape[1]), np.float32)
+    for k0 in range(0, a.shape[1], 8):
+        part = (a[:, k0:k0 + 8].astype(np.float64) @ b[k0:k0 + 8].astype(np.float64)).astype(np.float32)
+        acc = (acc + part).astype(np.float32)
+    return acc
+
+
 def forward(W, x):
     Wb = bf16(W)
     acts = [x]
     h = x
     for ly in range(4):
-        h = bf16(np.maximum(h @ Wb[ly], 0.0))
+        h = bf16(np.maximum(mm(h, Wb[ly]), 0.0))
         acts.append(h)
-    return h @ Wb[4][:, :3], acts
+    return mm(h, Wb[4][:, :3]), acts
 
 
 def backward(W, x, t):
@@ -39,12 +55,12 @@ def backward(W, x, t):
     Wb = bf16(W)
     g = bf16(2.0 * (out - t) / den / n)
     dW = np.zeros_like(W)
-    dW[4][:, :3] = acts[4].T @ g
-    gin = bf16((g @ Wb[4][:, :3].T) * (acts[4] > 0))
+    dW[4][:, :3] = mm(acts[4].T, g)
+    gin = bf16(mm(g, Wb[4][:, :3].T) * (acts[4] > 0))
     for ly in range(3, -1, -1):
-        dW[ly] = acts[ly].T @ gin
+        dW[ly] = mm(acts[ly].T, gin)
         if ly > 0:
-            gin = bf16((gin @ Wb[ly].T) * (acts[ly] > 0))
+            gin = bf16(mm(gin, Wb[ly].T) * (acts[ly] > 0))
     return loss, dW
 
 
@@ -71,10 +87,17 @@ def test_mlp_inference_matches_reference(hip, dev):
         got = hip.cache_eval(dev, bits(x))
         ref, _ = forward(W, x)
         scale = np.abs(ref).max() + 1e-6
-        err = np.abs(got - ref).max() / scale
-        print(f"n={n}: max |gpu - ref| / max |ref| = {err:.2e}")
+        rel = np.abs(got - ref) / scale
+        within = float(np.mean(rel <= 1e-5))
+        print(f"n={n}: max |gpu - ref| / max |ref| = {rel.max():.2e}, within 1e-5: {within:.4f}")
         assert np.all(np.isfinite(got))
-        assert err < 2e-2
+        # the inference kernel groups the K values of each 32-wide step in a
+        # permuted order (its activations stay in registers), so a few
+        # activations within rounding of a BF16 boundary may still round the
+        # other way (one BF16 ulp, 2^-8, in one input of the next layer); an
+        # operand-layout slip moves almost every output by > 1e-3
+        assert within >= 0.99
+        assert rel.max() < 1e-2
 
 
 def test_mlp_gradient_matches_reference(hip, dev):
@@ -87,12 +110,14 @@ def test_mlp_gradient_matches_reference(hip, dev):
     loss, grad = hip.cache_train(dev, bits(x), t, 0.0)  # lr 0: the step leaves the weights unchanged
     rloss, rgrad = backward(W, x, t)
     print(f"loss gpu {loss:.6f} ref {rloss:.6f}")
-    assert abs(loss - rloss) <= 2e-3 * abs(rloss)
+    assert abs(loss - rloss) <= 1e-5 * abs(rloss)
     for ly in range(5):
         a, b = grad[ly], rgrad[ly]
         rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-12)
         print(f"layer {ly}: |dW - ref| / |ref| = {rel:.2e}")
-        assert rel < 3e-2
+        # the reference sums in the MFMA's order: ~1e-7 (FP32 reassociation of the
+        # per-tile gradient atomics), measured 1.5e-7 on these inputs
+        assert rel < 1e-5
     assert np.all(grad[4][:, 3:] == 0.0)
     assert np.array_equal(hip.cache_weights(dev), W)  # step size 0
 
@@ -229,3 +254,33 @@ def test_cached_frames_on_several_streams(hip, monkeypatch, lanes):
     assert abs(a.mean() - ref.mean()) < 0.08 * ref.mean()
     assert abs(b.mean() - ref.mean()) < 0.08 * ref.mean()
     assert diff < 0.05
+
+
+def test_training_batches_stay_within_the_stored_records(hip, dev):
+    # 1000 records over 3 Adam steps: mini-batches of 334, 334 and the last 332
+    # (shade counts candidates past the 1000 it stores; training reads only those)
+    img, st = hip.render_cached(dev, 6, cache_depth=2, train_records=1000, train_steps=3, reset=True,
+                                weight_seed=3, max_depth=6, seed=4)
+    w = hip.cache_weights(dev)
+    print(st)
+    assert np.all(np.isfinite(w)) and np.all(np.isfinite(img))
+    assert 0 < st["records"] <= 1000 and st["steps"] == 6 * 3 and np.isfinite(st["last_loss"])
+
+
+def test_weights_read_after_session_frames(hip):
+    # a session's frames train on every lane's stream; reading the weights waits
+    # for all of them, so it sees the same weights as a read after wait()
+    from tinypt_amd import scenes
+
+    dev = scenes.triangle_room(96, 64, seed=2, tris=5000).to("hip")
+    sess = hip.ProgressiveRender(dev, cache_depth=2, train_records=1024, reset=True, weight_seed=7, max_depth=6,
+                                 seed=2)
+    sess.add(16)
+    w1 = hip.cache_weights(dev)
+    sess.wait()
+    w2 = hip.cache_weights(dev)
+    assert np.array_equal(w1, w2)
+    # a cached render on the same scene trains after the session's steps
+    _, st = hip.render_cached(dev, 4, cache_depth=2, train_records=1024, max_depth=6, seed=3)
+    assert st["steps"] == 16 * 4 + 4 * 4
+    sess.close()

@@ -135,6 +135,11 @@ struct tpt_hip_scene {
                                         // batches use up to three pools
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {1, 1, 1}; // finite geometry (radiance-cache encoding)
     CacheState cache;
+    // recorded after the latest training step of the scene's network, whichever
+    // cached render or session enqueued it: every inference and training waits on
+    // it, so the network's steps never overlap (they share the gradient buffer)
+    hipEvent_t train_ev = nullptr;
+    bool train_any = false;
     ~tpt_hip_scene() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (stream2) (void)hipStreamSynchronize(stream2);
@@ -149,6 +154,7 @@ struct tpt_hip_scene {
             if (x) (void)hipStreamDestroy(x);
         if (arena) (void)hipFree(arena);
         if (trace_counters) (void)hipFree(trace_counters);
+        if (train_ev) (void)hipEventDestroy(train_ev);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1046,7 +1052,7 @@ void cache_set_weights(tpt_hip_scene *h, const float *w) {
                 wb[i] = b;
                 wt[ly * 4096 + out * 64 + in] = b;
             }
-    HIP_OK(hipStreamSynchronize(h->stream));
+    sync_lanes(h); // frames on any lane may still train the network
     HIP_OK(hipMemcpy(c.W.w, wf.data(), 4 * kNrcParams, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c.W.wt, wt.data(), 2 * kNrcParams, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c.W.wb, wb.data(), 2 * kNrcParams, hipMemcpyHostToDevice));
@@ -1125,22 +1131,22 @@ struct FrameSetup {
 };
 constexpr int kFrameBlock = 256;
 
-// Orders frames that run on different streams: `train` is recorded after the
-// latest frame's training (the next inference reads the weights it wrote),
-// `acc` after its accumulation (the per-pixel sums are shared and added in
-// frame order), `start` where the other lanes' streams join the scene's stream.
+// Orders frames that run on different streams: `acc` is recorded after the
+// latest frame's accumulation (the per-pixel sums are shared and added in frame
+// order), `start` where the other lanes' streams join the scene's stream.  The
+// training order is the scene's (tpt_hip_scene::train_ev).
 struct FrameChain {
-    hipEvent_t train = nullptr, acc = nullptr, start = nullptr;
+    hipEvent_t acc = nullptr, start = nullptr;
     bool any = false; // a frame has been enqueued: later frames wait on the events
     FrameChain() = default;
     FrameChain(const FrameChain &) = delete;
     FrameChain &operator=(const FrameChain &) = delete;
     void create() { // each missing event (a failed call leaves the others to the destructor)
-        for (hipEvent_t *e : {&train, &acc, &start})
+        for (hipEvent_t *e : {&acc, &start})
             if (!*e) HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     ~FrameChain() {
-        for (hipEvent_t e : {train, acc, start})
+        for (hipEvent_t e : {acc, start})
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -1217,6 +1223,7 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[k
     const DScene &S = h->S;
     CacheState &c = h->cache;
     ch.create();
+    if (fs.use_cache && !h->train_ev) HIP_OK(hipEventCreateWithFlags(&h->train_ev, hipEventDisableTiming));
     if (fs.lanes > 1) { // the other lanes start after the work already queued on the first
         HIP_OK(hipEventRecord(ch.start, h->stream));
         for (int l = 1; l < fs.lanes; l++) HIP_OK(hipStreamWaitEvent(lane_stream(h, l), ch.start, 0));
@@ -1244,7 +1251,9 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[k
             cur = 1 - cur;
         }
         if (fs.use_cache) {
-            if (ch.any) HIP_OK(hipStreamWaitEvent(st, ch.train, 0));
+            // inference reads the weights of the scene's latest training step, and
+            // this frame's training must follow it (any session or cached render)
+            if (h->train_any) HIP_OK(hipStreamWaitEvent(st, h->train_ev, 0));
             if (clear_loss && f + 1 == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
             if (e) HIP_OK(hipEventRecord(e[0], st));
             nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, fs.pixels, c.W.wt, cb.pred, st);
@@ -1255,11 +1264,13 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[k
             // train_steps Adam steps on consecutive mini-batches of the frame's records
             const int K = fs.cp.train_steps, mb = (fs.cp.train_records + K - 1) / K;
             for (int k = 0; k < K; k++) {
-                nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, k * mb, mb, c.W, st);
+                nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, fs.cp.train_records, k * mb, mb,
+                                 c.W, st);
                 nrc_launch_adam(c.W, fs.cp.learning_rate, ++c.step, st);
             }
             if (e) HIP_OK(hipEventRecord(e[3], st));
-            HIP_OK(hipEventRecord(ch.train, st));
+            HIP_OK(hipEventRecord(h->train_ev, st));
+            h->train_any = true;
         }
         if (ch.any) HIP_OK(hipStreamWaitEvent(st, ch.acc, 0));
         launch_frame_accumulate(P, fs.pixels, fs.fp64, f == 0, st);
@@ -1442,7 +1453,7 @@ int tpt_hip_cache_weights(tpt_hip_scene *h, const float *set, float *get) {
         if (set) cache_set_weights(h, set);
         else if (h->cache.step == 0 && !get) cache_init_weights(h, 0);
         if (get) {
-            HIP_OK(hipStreamSynchronize(h->stream));
+            sync_lanes(h); // frames on any lane may still train the network
             HIP_OK(hipMemcpy(get, h->cache.W.w, 4 * kNrcParams, hipMemcpyDeviceToHost));
         }
         return TPT_OK;
@@ -1457,7 +1468,7 @@ void cache_stage(tpt_hip_scene *h, const uint16_t *inputs, const float *targets,
     cache_alloc_weights(h);
     cache_alloc_buffers(h, (size_t)std::max(n, 64), std::max(n, 64), 0);
     CacheBufs &b = h->cache.buf[0];
-    HIP_OK(hipStreamSynchronize(h->stream));
+    sync_lanes(h); // frames on any lane may still use lane 0's buffers or train the network
     HIP_OK(hipMemcpy(b.feat, inputs, (size_t)n * kNrcInputs * 2, hipMemcpyHostToDevice));
     if (targets) {
         std::vector<float4> t(n);
@@ -1502,7 +1513,7 @@ int tpt_hip_cache_train(tpt_hip_scene *h, const uint16_t *inputs, const float *t
         const CacheBufs &b = c.buf[0];
         hipStream_t st = h->stream;
         HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
-        nrc_launch_train(b.feat, nullptr, b.target, b.counters + 1, 0, n, c.W, st);
+        nrc_launch_train(b.feat, nullptr, b.target, b.counters + 1, n, 0, n, c.W, st);
         HIP_OK(hipGetLastError());
         if (grad) HIP_OK(hipMemcpyAsync(grad, c.W.grad, 4 * kNrcParams, hipMemcpyDeviceToHost, st));
         if (loss) HIP_OK(hipMemcpyAsync(loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));

@@ -209,12 +209,14 @@ static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per w
 // W[l][in][out] layout) with atomics; the mean loss is added to *loss.
 __global__ __launch_bounds__(64) void k_nrc_train(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
                                                   const float4 *__restrict__ target, const uint32_t *__restrict__ n_dev,
-                                                  int base, int n_max, const uint16_t *__restrict__ wt,
+                                                  int cap, int base, int n_max, const uint16_t *__restrict__ wt,
                                                   const uint16_t *__restrict__ wb, float *__restrict__ grad,
                                                   float *__restrict__ loss) {
     __shared__ __attribute__((aligned(16))) uint16_t act[5][64 * kRow]; // layer inputs x0..x4
     __shared__ __attribute__((aligned(16))) uint16_t g[2][64 * kRow];   // output gradients (ping-pong)
-    const int n = min(max((int)*n_dev - base, 0), n_max); // records base .. base + n - 1
+    // records base .. base + n - 1 of the first min(*n_dev, cap): the producer keeps
+    // counting candidates past the `cap` records it stores
+    const int n = min(max(min((int)*n_dev, cap) - base, 0), n_max);
     if (idx) idx += base;
     target += base;
     const int l = threadIdx.x;
@@ -347,10 +349,10 @@ void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t 
     hipLaunchKernelGGL(k_nrc_infer<kInferTiles>, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
 }
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
-                      int base, int n_max, const NrcWeights &W, hipStream_t st) {
+                      int cap, int base, int n_max, const NrcWeights &W, hipStream_t st) {
     if (n_max <= 0) return;
-    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64), 0, st, feats, idx, target, n_dev, base, n_max,
-                       W.wt, W.wb, W.grad, W.loss);
+    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64), 0, st, feats, idx, target, n_dev, cap, base,
+                       n_max, W.wt, W.wb, W.grad, W.loss);
 }
 void nrc_launch_adam(const NrcWeights &W, float lr, int step, hipStream_t st) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
