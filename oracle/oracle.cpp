@@ -657,7 +657,7 @@ struct oracle_scene {
         }
         }
     }
-    float micro_D(const Lobe &b, V3 n, V3 h) const {
+    static float micro_D(const Lobe &b, V3 n, V3 h) {
         if (b.type == TPT_BXDF_GLOSSY_BLINN) { // material.cpp:70-73
             float cb = dot(h, n);
             return (cb <= 0) ? 0 : (b.param + 2) * (.5f * INV_PI_F) * std::pow(cb, b.param + 1);
@@ -668,7 +668,7 @@ struct oracle_scene {
         float s2 = 1 - c2;
         return b.alpha2 * INV_PI_F * cb / square(b.alpha2 * c2 + s2);
     }
-    float micro_G(const Lobe &b, V3 l, V3 n, V3 v) const {
+    static float micro_G(const Lobe &b, V3 l, V3 n, V3 v) {
         if (b.type == TPT_BXDF_GLOSSY_BLINN) { // material.cpp:37-56
             auto G1 = [&](V3 w) {
                 float ct = dot(w, n);
@@ -689,6 +689,24 @@ struct oracle_scene {
             return 2 / (1 + std::sqrt(1 + b.alpha2 * tan2));
         };
         return G1(l) * G1(v);
+    }
+    // microfacet normal: BlinnPhongModel::sample_h (material.cpp:58-66) and
+    // TrowbridgeReitzModel::sample_h (:88-99)
+    static V3 micro_sample_h(const Lobe &b, V3 n, RandEngine &rng) {
+        float cb, sb, phi;
+        if (b.type == TPT_BXDF_GLOSSY_BLINN) {
+            cb = std::pow(rng.random(), 1 / (b.param + 2));
+            sb = std::sqrt(1 - cb * cb);
+            phi = rng.random(2.f * PI_F);
+        } else {
+            float u = rng.random();
+            float c2 = (1 - u) / (1 - (1 - b.alpha2) * u);
+            cb = std::sqrt(c2);
+            sb = std::sqrt(1 - c2);
+            phi = rng.random(2 * PI_F);
+        }
+        Rot r = from_direction(n);
+        return rot_apply(r, V3{std::cos(phi) * sb, std::sin(phi) * sb, cb});
     }
     V3 lambert_sample(const Hit &hit, RandEngine &rng) const { // material.cpp:23-33
         float phi = rng.random(2.f * PI_F);
@@ -716,22 +734,8 @@ struct oracle_scene {
             float Re = R0 + (1 - R0) * c * c * c * c * c;
             return (rng.random() < Re) ? refl : refr;
         }
-        default: { // Glossy::sample (material.cpp:136-144) with sample_h (:58-66 / :88-99)
-            V3 n = hit.normal;
-            float cb, sb, phi;
-            if (b.type == TPT_BXDF_GLOSSY_BLINN) {
-                cb = std::pow(rng.random(), 1 / (b.param + 2));
-                sb = std::sqrt(1 - cb * cb);
-                phi = rng.random(2.f * PI_F);
-            } else {
-                float u = rng.random();
-                float c2 = (1 - u) / (1 - (1 - b.alpha2) * u);
-                cb = std::sqrt(c2);
-                sb = std::sqrt(1 - c2);
-                phi = rng.random(2 * PI_F);
-            }
-            Rot r = from_direction(n);
-            V3 h = rot_apply(r, V3{std::cos(phi) * sb, std::sin(phi) * sb, cb});
+        default: { // Glossy::sample (material.cpp:136-144)
+            V3 h = micro_sample_h(b, hit.normal, rng);
             V3 out = reflect(ray.dir, h);
             if (!same_side(out, hit.normal)) return lambert_sample(hit, rng);
             return out;
@@ -1546,6 +1550,50 @@ int oracle_bxdf_sample(const oracle_scene *s, int32_t material, const float *rd,
     od[0] = o.x;
     od[1] = o.y;
     od[2] = o.z;
+    return 0;
+}
+
+static Lobe glossy_lobe(int32_t type, float shininess) {
+    Lobe b;
+    b.type = type;
+    b.param = shininess;
+    if (type == TPT_BXDF_GLOSSY_GGX) b.alpha2 = 2 / (shininess + 2); // material.cpp:118
+    return b;
+}
+
+int oracle_micro_terms(int32_t type, float shininess, const float *n3, const float *l3, const float *v3,
+                       const float *h3, float *D, float *G) {
+    if (type != TPT_BXDF_GLOSSY_BLINN && type != TPT_BXDF_GLOSSY_GGX) return -1;
+    Lobe b = glossy_lobe(type, shininess);
+    V3 n{n3[0], n3[1], n3[2]}, l{l3[0], l3[1], l3[2]}, v{v3[0], v3[1], v3[2]}, h{h3[0], h3[1], h3[2]};
+    *D = oracle_scene::micro_D(b, n, h);
+    *G = oracle_scene::micro_G(b, l, n, v);
+    return 0;
+}
+
+int oracle_micro_sample_h(int32_t type, float shininess, const float *n3, uint64_t seed, int32_t count, float *h3) {
+    if (type != TPT_BXDF_GLOSSY_BLINN && type != TPT_BXDF_GLOSSY_GGX) return -1;
+    Lobe b = glossy_lobe(type, shininess);
+    V3 n{n3[0], n3[1], n3[2]};
+    RandEngine rng(seed);
+    for (int32_t i = 0; i < count; i++) {
+        V3 h = oracle_scene::micro_sample_h(b, n, rng);
+        h3[3 * i] = h.x;
+        h3[3 * i + 1] = h.y;
+        h3[3 * i + 2] = h.z;
+    }
+    return 0;
+}
+
+int oracle_camera_ray(const oracle_scene *s, float x, float y, uint64_t seed, float *org3, float *dir3) {
+    RandEngine rng(seed);
+    Ray r = s->shoot(x, y, rng);
+    org3[0] = r.org.x;
+    org3[1] = r.org.y;
+    org3[2] = r.org.z;
+    dir3[0] = r.dir.x;
+    dir3[1] = r.dir.y;
+    dir3[2] = r.dir.z;
     return 0;
 }
 

@@ -70,6 +70,16 @@ int oracle_bxdf_eval(const oracle_scene *s, int32_t material, const float *ray_d
 int oracle_bxdf_sample(const oracle_scene *s, int32_t material, const float *ray_dir3, const float *normal3,
                        int32_t into, uint64_t seed, float *out_dir3);
 
+/* Microfacet terms of a Glossy lobe (cpu/material.cpp:35-111): type TPT_BXDF_GLOSSY_BLINN
+ * (BlinnPhongModel(shininess)) or TPT_BXDF_GLOSSY_GGX (TrowbridgeReitzModel(2 / (shininess + 2))):
+ * D(n, h) (also pdf_h) and G(l, n, v). */
+int oracle_micro_terms(int32_t type, float shininess, const float *n3, const float *l3, const float *v3,
+                       const float *h3, float *D, float *G);
+/* `count` microfacet normals from sample_h (material.cpp:58-66, 88-99), one RandEngine(seed) stream. */
+int oracle_micro_sample_h(int32_t type, float shininess, const float *n3, uint64_t seed, int32_t count, float *h3);
+/* Camera::shoot_ray(x, y) (cpu/camera.cpp:23-34) with RandEngine(seed) for the lens. */
+int oracle_camera_ray(const oracle_scene *s, float x, float y, uint64_t seed, float *org3, float *dir3);
+
 #ifdef __cplusplus
 }
 #endif

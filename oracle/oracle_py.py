@@ -61,6 +61,9 @@ def lib():
         L.oracle_light_sample.argtypes = [vp, i32, vp, u64, vp, vp]
         L.oracle_bxdf_eval.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
         L.oracle_bxdf_sample.argtypes = [vp, i32, vp, vp, i32, u64, vp]
+        L.oracle_micro_terms.argtypes = [i32, ctypes.c_float, vp, vp, vp, vp, vp, vp]
+        L.oracle_micro_sample_h.argtypes = [i32, ctypes.c_float, vp, u64, i32, vp]
+        L.oracle_camera_ray.argtypes = [vp, ctypes.c_float, ctypes.c_float, u64, vp, vp]
         _lib = L
     return _lib
 
@@ -139,3 +142,31 @@ def scene_width(scene):
 def scene_height(scene):
     buf = (ctypes.c_int32 * 14).from_address(scene._desc_ptr())
     return int(buf[13])
+
+
+BLINN, GGX = 3, 4  # TPT_BXDF_GLOSSY_BLINN, TPT_BXDF_GLOSSY_GGX
+
+
+def micro_terms(kind, shininess, n, l, v, h):
+    """(D(n, h), G(l, n, v)) of a Glossy lobe's microfacet model."""
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (n, l, v, h)]
+    D, G = ctypes.c_float(), ctypes.c_float()
+    rc = lib().oracle_micro_terms(kind, shininess, *[x.ctypes.data for x in a], ctypes.byref(D), ctypes.byref(G))
+    assert rc == 0
+    return D.value, G.value
+
+
+def micro_sample_h(kind, shininess, n, seed, count):
+    """(count, 3) microfacet normals from one RandEngine(seed) stream."""
+    n = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.empty((count, 3), dtype=np.float32)
+    assert lib().oracle_micro_sample_h(kind, shininess, n.ctypes.data, seed, count, out.ctypes.data) == 0
+    return out
+
+
+def camera_ray(oracle_scene, x, y, seed=0):
+    """(origin, direction) of Camera::shoot_ray(x, y) with RandEngine(seed) for the lens."""
+    o = np.empty(3, dtype=np.float32)
+    d = np.empty(3, dtype=np.float32)
+    lib().oracle_camera_ray(oracle_scene.h, x, y, seed, o.ctypes.data, d.ctypes.data)
+    return o, d
