@@ -449,13 +449,14 @@ def city(width=1920, height=1080, seed=21, target_tris=6_000_000, point_lights=1
     return _C.Scene(cam, [mesh], lights, bg)
 
 
-CONFIG4 = dict(width=1920, height=1080, spp=1024, max_depth=8, rr_depth=5)
-
-
-# presets: (spp, max_depth, rr_depth)
+# Workload presets.  Only the two-spheres box asks for Russian roulette (after
+# depth 5); the others name a maximum depth and no roulette (rr_depth -1), and the
+# city names no depth, so it takes the triangle room's 8.
 CONFIG0 = dict(width=512, height=384, spp=64, max_depth=10, rr_depth=5)
-CONFIG1 = dict(width=1024, height=768, spp=256, max_depth=8, rr_depth=5)
-CONFIG2 = dict(count=64, tris=(2000, 500000), side=(128, 1024), spp=(16, 256), max_depth=8, rr_depth=5)
+CONFIG1 = dict(width=1024, height=768, spp=256, max_depth=8, rr_depth=-1)
+CONFIG2 = dict(count=64, tris=(2000, 500000), side=(128, 1024), spp=(16, 256), max_depth=8, rr_depth=-1)
+CONFIG3 = dict(width=1920, height=1080, max_depth=12, rr_depth=-1)
+CONFIG4 = dict(width=1920, height=1080, spp=1024, max_depth=8, rr_depth=-1)
 
 
 def batch_specs(count=64, seed=2024, tris=(2000, 500000), side=(128, 1024), spp=(16, 256)):
