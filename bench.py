@@ -1,13 +1,15 @@
 """Benchmark: the plain HIP wavefront path tracer on the seeded ~265k-triangle
-room (1024x768, 256 spp, max depth 8, Russian roulette after depth 5).
+room (1024x768, 256 spp, max depth 8, no Russian roulette), and the learned
+radiance cache against the plain tracer at equal relMSE on the textured room
+(1920x1080, depth 12) against a 16,384-spp FP64 render.
 
-One step = one full render of that workload (201,326,592 paths) through
+One step = one full render of the room (201,326,592 paths) through
 PathTracer("hip").render; the scene is resident in HBM before the timed region.
-Prints ONE JSON line with Mpaths/s, the extend kernel's roofline figure
-(algorithmic bytes from a counting render with the same seed / HIP-event time
-of the extend launches) and the CPU oracle timed on a bounded sample.
+Prints ONE JSON line with Mpaths/s, the extend kernel's roofline (VALU issue:
+the kernel's bound; its HBM bytes from counters and its algorithmic bytes beside
+it), the CPU oracle timed on a bounded sample, and the radiance-cache rows.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--spp S] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--spp S] [--no-cpu] [--no-cache]
 
 Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
@@ -19,11 +21,9 @@ Other workloads (one JSON line each, same fields):
                       one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
   --workload city     the ~6M-triangle city (HDR sky CDF, sun, 1000 point lights),
-                      1920x1080 at 1024 spp: plain tracer (reference image), batched pipeline
-                      and radiance cache
-  --workload cache    the textured room at 1920x1080, depth 12: plain tracer vs the
-                      learned radiance cache at equal relMSE against a plain FP64
-                      reference of --ref-spp (16384) samples per pixel
+                      1920x1080 at 1024 spp: plain tracer, batched pipeline
+                      and radiance cache, against a 16,384-spp FP64 render
+  --workload cache    the radiance-cache comparison alone
 """
 import argparse
 import json
@@ -37,16 +37,30 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpaths/s + equal-time relMSE vs 16k-spp oracle, plain vs radiance-cache"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of an MI355X: 256 CUs x 4 SIMDs of 16 lanes, one wave64 VALU
+# instruction per SIMD every 4 cycles, 2.4 GHz peak engine clock
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 # extend algorithmic bytes: per ray 32 (ray) + 4 (queue index) + 16 (hit), per triangle record 48,
 # per node visit the node record's size (64 B for the quantized 4-wide layout; from scene_info)
 RAY_BYTES, TRI_BYTES = 52, 48
-# HBM traffic of one extend launch of the default workload (room 1024x768, 256 spp), from two
-# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over
-# `bench.py --steps 1 --warmup 0 --no-cpu --no-count`: KiB per dispatch averaged over the 117 extend
-# dispatches, scaled by what the counters read for a 1 GiB device copy (`bench.py --copy-calibration`:
-# FETCH_SIZE reads half the copied bytes -> x2.0, WRITE_SIZE all of them -> x1.0).  Counters cannot be
-# read inside this process, so the figure is the profiled one; it is reported only for that workload.
-EXTEND_TRAFFIC_KIB = {"fetch": 1236983.0, "fetch_scale": 2.0, "write": 258748.9, "write_scale": 1.0}
+# Hardware counters of the extend kernel (k_extend_chunk), averaged per dispatch over one render
+# with dispatches serialised, from rocprofv3 --pmc passes:
+#   valu_insts     SQ_INSTS_VALU (wave64 VALU instructions issued);
+#   thread_cycles / active_inst  SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU (their ratio: active
+#                  lanes per VALU instruction);
+#   fetch_kib / write_kib  FETCH_SIZE / WRITE_SIZE (KiB), scaled by what the counters read for a
+#                  1 GiB device copy (`bench.py --copy-calibration`: FETCH_SIZE reads half the copied
+#                  bytes -> x2.0, WRITE_SIZE all of them -> x1.0).
+# Counters cannot be read inside this process, so these are the profiled figures, reported only for
+# the workload they were taken on.
+# Room 1024x768, 256 spp, depth 8: `bench.py --steps 1 --warmup 0 --no-cpu --no-count --no-cache`
+# (117 dispatches).
+EXTEND_COUNTERS = {"valu_insts": 1119544484, "thread_cycles": 37428053039, "active_inst": 1124401248,
+                   "fetch_kib": 1643711.6, "write_kib": 306683.7}
+# City 1920x1080, depth 8: one 16-spp render (18 dispatches, waves of 8 samples per pixel).
+CITY_EXTEND_COUNTERS = {"valu_insts": 1380013100, "thread_cycles": 49691613932, "active_inst": 1383996680,
+                        "fetch_kib": 3085776.4, "write_kib": 262941.7}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 DEFAULT_ROOM = (1024, 768, 256)
 
 
@@ -64,9 +78,28 @@ def copy_calibration(mib=1024):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
 
 
+def host_cpu():
+    """(model name, threads this process may use) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    return model, threads
+
+
 def cpu_baseline(spp=64, rows=256, width=1024, height=768):
-    """The CPU oracle (C++/OpenMP restatement of the reference CPU renderer) on a
-    bounded strip of the same workload: `rows` rows of the 1024x768 image."""
+    """The CPU oracle (C++/OpenMP restatement of the reference CPU renderer, built
+    -O3 -march=x86-64-v3 -ffp-contract=off) on a bounded strip of the same
+    workload: `rows` rows of the 1024x768 image, depth 8, no roulette."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
 
@@ -74,13 +107,10 @@ def cpu_baseline(spp=64, rows=256, width=1024, height=768):
 
     host = scenes.triangle_room(width, height)
     oracle = O.OracleScene(host)
-    threads = os.cpu_count() or 1
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
+    model, threads = host_cpu()
     y0 = height // 2 - rows // 2
-    p = O.params(spp, max_depth=8, rr_depth=5, seed=1)
+    c = scenes.CONFIG1
+    p = O.params(spp, max_depth=c["max_depth"], rr_depth=c["rr_depth"], seed=1)
     t0 = time.perf_counter()
     oracle.render(height, p, y0, y0 + rows, threads=threads)
     dt = time.perf_counter() - t0
@@ -90,7 +120,9 @@ def cpu_baseline(spp=64, rows=256, width=1024, height=768):
         "unit": "Mpaths/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"triangle room rows {y0}-{y0 + rows} of 1024x768 at {spp} spp ({paths} paths, {dt:.1f} s)",
+        "sample": f"triangle room rows {y0}-{y0 + rows} of 1024x768 at {spp} spp, depth 8 ({paths} paths, {dt:.1f} s)",
+        "cpu": model,
+        "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
     }
 
 
@@ -104,10 +136,13 @@ def main():
     ap.add_argument("--height", type=int, default=768)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-cache", action="store_true", help="room: skip the radiance-cache rows")
+    ap.add_argument("--render-only", action="store_true",
+                    help="city: one render at --spp (and a counting render unless --no-count), for counter passes")
     ap.add_argument("--block-size", type=int, default=0, help="room: threads per workgroup (0 = the library's default)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache", "city"], default="room")
-    ap.add_argument("--ref-spp", type=int, default=16384, help="cache: samples per pixel of the reference")
+    ap.add_argument("--ref-spp", type=int, default=16384, help="cache, city: samples per pixel of the reference")
     ap.add_argument("--cache-suffix", type=int, default=3,
                     help="cache: training paths end in the cache this many vertices past their cache vertex "
                          "(0: they run to the maximum depth)")
@@ -128,7 +163,6 @@ def main():
         sys.exit(2)
 
     import tinypt_amd
-    from tinypt_amd import scenes
 
     if tinypt_amd.hip_device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
@@ -143,17 +177,90 @@ def main():
         bench_city(args)
         return
     if args.workload == "cache":
-        if args.width == 1024 and args.height == 768:
-            args.width, args.height = 1920, 1080
         bench_cache(args)
         return
+    bench_room(args)
+
+
+def extend_roofline(pt, host, info, kw, spp, ext_ms, count_stats, counters, streams=3):
+    """Roofline of the extend kernel.  Its bound is VALU issue (counters: ~87% of
+    the SIMD cycles issue VALU instructions while it runs alone), so `frac` is
+    the issue rate of the kernel alone (a one-stream scene handle, no other
+    kernel beside it) against the VALU issue peak; its HBM bytes (counters) and
+    the survey's algorithmic bytes are reported beside it.  `counters`: the
+    profiled per-dispatch counters of this workload, or None."""
+    c = count_stats
+    rays = c["rays_closest"]
+    launches = max(1, c["extend_launches"])
+    alg_bytes = rays * RAY_BYTES + c["nodes_visited"] * info["node_bytes"] + c["tris_tested"] * TRI_BYTES
+    # the same kernel without the other streams' co-running waves (a scene handle
+    # created with TPT_STREAMS=1, one untimed warm-up render, then one measured)
+    prev = os.environ.get("TPT_STREAMS")
+    os.environ["TPT_STREAMS"] = "1"
+    try:
+        dev1 = host.to("hip")
+    finally:
+        if prev is None:
+            os.environ.pop("TPT_STREAMS")
+        else:
+            os.environ["TPT_STREAMS"] = prev
+    pt.render(dev1, spp, **kw)
+    pt.render(dev1, spp, **kw)
+    iso = pt.last_stats()
+    del dev1
+    iso_s = iso["extend_ms"] / launches / 1e3
+    co_s = (sum(ext_ms) / len(ext_ms)) / launches / 1e3
+    insts = traffic = lanes = None
+    if counters is not None:
+        k = counters
+        insts = k["valu_insts"]
+        traffic = (k["fetch_kib"] * FETCH_SCALE + k["write_kib"] * WRITE_SCALE) * 1024
+        lanes = k["thread_cycles"] / k["active_inst"]
+    valu = insts / iso_s / 1e9 if insts else None
+    alg = alg_bytes / launches
+    roofline = {
+        "bound": "valu",
+        "achieved": round(valu, 1) if valu else None,
+        "peak": VALU_PEAK_GINST,
+        "unit": "Gwaveinst/s",
+        "frac": round(valu / VALU_PEAK_GINST, 4) if valu else None,
+        "traffic": round(traffic) if traffic else None,
+        "kernel": "k_extend_chunk",
+        "avg_launch_ms": round(iso_s * 1e3, 4),
+        "valu_insts_per_launch": round(insts) if insts else None,
+        "active_lanes_per_inst": round(lanes, 1) if lanes else None,
+        "hbm": {"achieved": round(traffic / iso_s / 1e9, 1) if traffic else None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(traffic / iso_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                "source": "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch / isolated launch time"},
+        "algorithmic": {"bytes_per_launch": round(alg), "achieved": round(alg / iso_s / 1e9, 1), "unit": "GB/s",
+                        "per_ray": "52 + 64 N_node + 48 N_tri (node and triangle fetches, mostly L2 / MALL hits)"},
+        # the timed renders run pipeline waves at once on three streams, so each extend
+        # launch shares the CUs with the other waves' shading
+        "co_scheduled": {"streams": streams, "avg_launch_ms": round(co_s * 1e3, 4),
+                         "achieved": round(insts / co_s / 1e9, 1) if insts else None},
+    }
+    extra = {
+        "extend_mrays_per_s": round(rays / (iso["extend_ms"] / 1e3) / 1e6, 1),
+        "nodes_per_ray": round(c["nodes_visited"] / max(1, rays), 2),
+        "tris_per_ray": round(c["tris_tested"] / max(1, rays), 2),
+        "extend_share": round(iso["extend_ms"] / iso["render_ms"], 3),
+    }
+    return roofline, extra
+
+
+def bench_room(args):
+    """The triangle room: one step = one render of 1024x768 at 256 spp, depth 8."""
+    import tinypt_amd
+    from tinypt_amd import scenes
+
     t0 = time.perf_counter()
     host = scenes.triangle_room(args.width, args.height)
     dev = host.to("hip")
     setup_s = time.perf_counter() - t0
     info = tinypt_amd.scene_info(dev)
     pt = tinypt_amd.PathTracer("hip")
-    kw = dict(max_depth=8, rr_depth=5, seed=1)
+    c1 = scenes.CONFIG1
+    kw = dict(max_depth=c1["max_depth"], rr_depth=c1["rr_depth"], seed=1)
     if args.block_size:
         kw["block_size"] = args.block_size
     print(f"# scene ready in {setup_s:.1f} s: {info}", file=sys.stderr, flush=True)
@@ -180,57 +287,16 @@ def main():
         # counting render: identical paths (same seed), extend counts node / triangle fetches
         pt.render(dev, args.spp, count_traffic=True, **kw)
         c = pt.last_stats()
-        rays = c["rays_closest"]
-        alg_bytes = rays * RAY_BYTES + c["nodes_visited"] * info["node_bytes"] + c["tris_tested"] * TRI_BYTES
-        ext_time_s = (sum(ext_ms) / len(ext_ms)) / 1e3
-        achieved = alg_bytes / ext_time_s / 1e9
-        traffic = None
-        if (args.width, args.height, args.spp) == DEFAULT_ROOM:
-            t = EXTEND_TRAFFIC_KIB
-            traffic = round((t["fetch"] * t["fetch_scale"] + t["write"] * t["write_scale"]) * 1024)
-        # the same kernel without the other streams' co-running waves (a scene handle
-        # created with TPT_STREAMS=1, one untimed warm-up render, then one measured)
-        prev = os.environ.get("TPT_STREAMS")
-        os.environ["TPT_STREAMS"] = "1"
-        try:
-            dev1 = host.to("hip")
-        finally:
-            if prev is None:
-                os.environ.pop("TPT_STREAMS")
-            else:
-                os.environ["TPT_STREAMS"] = prev
-        pt.render(dev1, args.spp, **kw)
-        pt.render(dev1, args.spp, **kw)
-        iso = pt.last_stats()
-        iso_ms = iso["extend_ms"] / max(1, c["extend_launches"])
-        iso_achieved = alg_bytes / max(1, c["extend_launches"]) / (iso_ms / 1e3) / 1e9
-        del dev1
-        roofline = {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "k_extend",
-            "node_bytes": info["node_bytes"],
-            "bytes_per_launch": alg_bytes / max(1, c["extend_launches"]),
-            "avg_launch_ms": (sum(ext_ms) / len(ext_ms)) / max(1, c["extend_launches"]),
-            # the timed renders run pipeline waves at once on several streams (three by
-            # default), so each extend launch shares the CUs with the others' shading; alone:
-            "isolated": {"streams": 1, "achieved": round(iso_achieved, 1),
-                         "frac": round(iso_achieved / HBM_PEAK_GBS, 4), "avg_launch_ms": iso_ms},
-        }
-        extra = {
-            "mrays_per_s": round((rays + c["rays_shadow"]) / (ms_per_step / 1e3) / 1e6, 1),
-            # extend alone (the one-stream render): closest-hit rays per second of extend time,
-            # and the share of that render's time spent in extend
-            "extend_mrays_per_s": round(rays / (iso["extend_ms"] / 1e3) / 1e6, 1),
-            "nodes_per_ray": round(c["nodes_visited"] / max(1, rays), 2),
-            "tris_per_ray": round(c["tris_tested"] / max(1, rays), 2),
-            "extend_share": round(iso["extend_ms"] / iso["render_ms"], 3),
-        }
+        profiled = EXTEND_COUNTERS if (args.width, args.height, args.spp) == DEFAULT_ROOM else None
+        roofline, extra = extend_roofline(pt, host, info, kw, args.spp, ext_ms, c, profiled)
+        extra = {"mrays_per_s": round((c["rays_closest"] + c["rays_shadow"]) / (ms_per_step / 1e3) / 1e6, 1), **extra}
+        print(f"# roofline: {roofline}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    cache = None
+    if not args.no_cache:
+        cache = cache_comparison(args, "16,64,256")
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -245,7 +311,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"triangle_room_{info['triangles']}tri_{args.width}x{args.height}_{args.spp}spp_depth8_rr5",
+            "workload": f"triangle_room_{info['triangles']}tri_{args.width}x{args.height}_{args.spp}spp_depth8",
             "tracer": "plain HIP wavefront",
             "paths_per_step": paths,
         },
@@ -253,6 +319,8 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    if cache:
+        out["radiance_cache_textured_room"] = cache
     print(json.dumps(out))
 
 
@@ -288,7 +356,8 @@ def bench_batch(args):
     print(f"# {len(devs)} scenes ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     spps = [sp["spp"] for sp in specs]
     seeds = [sp["seed"] for sp in specs]
-    kw = dict(max_depth=8, rr_depth=5)
+    c2 = scenes.CONFIG2
+    kw = dict(max_depth=c2["max_depth"], rr_depth=c2["rr_depth"])
     pt = tinypt_amd.PathTracer("hip")
     t = time.perf_counter()
     pt.render_batch(devs, spps, seeds, **kw)  # autotunes queue capacity and workgroup size
@@ -310,7 +379,7 @@ def bench_batch(args):
     done = sorted(st["scene_ms"])
     out = base_line(paths / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
         "workload": (f"batch_{len(devs)}_rooms_{ranges['tris'][0] // 1000}k-{ranges['tris'][1] // 1000}k_tris_"
-                     f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8_rr5"),
+                     f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8"),
         "tracer": "plain HIP wavefront, three path pools on three streams, captured pipeline wave per pool replayed",
         "paths_per_step": paths,
         "queue_capacity": cap,
@@ -333,24 +402,27 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
-def bench_cache(args):
-    """The textured room (RGBA8 albedo + alpha cut-out + FP32 bump maps), depth 12,
-    plain tracer vs the learned radiance cache, both against a plain FP64 reference
-    of --ref-spp samples per pixel.  Plain relMSE falls as c / time (unbiased
-    Monte Carlo), so the plain tracer's time to reach each cached render's relMSE
-    is c / relMSE; the ratio to the cached render's time is the equal-error speedup."""
+def cache_comparison(args, frame_counts, width=1920, height=1080):
+    """The textured room (RGBA8 albedo + alpha cut-out + FP32 bump maps), 1920x1080,
+    depth 12: the plain tracer and the learned radiance cache, both against a plain
+    FP64 reference of --ref-spp samples per pixel.  Plain relMSE falls as c / time
+    (unbiased Monte Carlo), so the plain tracer's time to reach each cached
+    render's relMSE is c / relMSE; the ratio to the cached render's time is the
+    equal-error speedup.  Every frame-count row is reported."""
     import numpy as np
 
     import tinypt_amd
     from tinypt_amd import scenes
 
-    W, H = args.width, args.height
+    c3 = scenes.CONFIG3
+    W, H = width, height
     dev = scenes.textured_room(W, H).to("hip")
     pt = tinypt_amd.PathTracer("hip")
-    kw = dict(max_depth=12, rr_depth=5)
+    kw = dict(max_depth=c3["max_depth"], rr_depth=c3["rr_depth"])
     t = time.perf_counter()
     ref = pt.render(dev, args.ref_spp, seed=777, hdr=True, fp64=True, **kw).raw()
-    print(f"# reference {args.ref_spp} spp in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    ref_s = time.perf_counter() - t
+    print(f"# textured room reference {args.ref_spp} spp in {ref_s:.1f} s", file=sys.stderr, flush=True)
 
     def rel(x):
         return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
@@ -366,7 +438,7 @@ def bench_cache(args):
     c = sum(e * dt for _, dt, e in plain) / len(plain)  # relMSE * time of the plain tracer
     cached = []
     tinypt_amd.render_cached(dev, 2, cache_depth=2, reset=True, seed=3, suffix_len=args.cache_suffix, **kw)  # warm-up
-    for frames in [int(x) for x in args.cache_frames.split(",")]:
+    for frames in [int(x) for x in frame_counts.split(",")]:
         t = time.perf_counter()
         img, st = tinypt_amd.render_cached(dev, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3,
                                            reset=True, weight_seed=5, seed=1, suffix_len=args.cache_suffix, **kw)
@@ -379,29 +451,50 @@ def bench_cache(args):
                            mean_bias=round(float(img.mean() / ref.mean() - 1.0), 4)))
         print(f"# cached {frames:4d} frames: {dt * 1e3:8.1f} ms relMSE {e:.5f} -> equal-error speedup "
               f"{cached[-1]['speedup']:.2f} (bias {cached[-1]['mean_bias']:+.4f})", file=sys.stderr, flush=True)
-    best = max(cached, key=lambda r: r["speedup"])
-    paths = W * H * best["frames"]
-    out = base_line(paths / (best["ms"] / 1e3) / 1e6, best["ms"], args, {
-        "workload": f"textured_room_264612tri_{W}x{H}_depth12_rr5_rgba8_alpha_bump",
-        "tracer": (f"radiance cache (5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, cache after 2 bounces, "
-                   f"training suffixes end in the cache after {args.cache_suffix} more vertices)"),
+    return {
+        "workload": f"textured_room_264612tri_{W}x{H}_depth12_rgba8_alpha_bump",
+        "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation ({ref_s:.1f} s)",
+        "cache": ("5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, 4 steps/frame, cache after 2 bounces, "
+                  f"training suffixes end in the cache after {args.cache_suffix} more vertices"),
+        "plain_relmse_times_s": round(c, 5),
+        "plain": [dict(spp=s, ms=round(dt * 1e3, 1), relmse=round(e, 5)) for s, dt, e in plain],
+        "cached": cached,
+        # the row of the most frames: the longest, least noisy comparison
+        "equal_relmse_speedup": cached[-1]["speedup"],
+        "equal_relmse_speedup_frames": cached[-1]["frames"],
+    }
+
+
+def bench_cache(args):
+    """The radiance-cache comparison alone (every frame-count row); `value` is the
+    cached tracer's Mpaths/s at the row of the most frames."""
+    W, H = 1920, 1080
+    if (args.width, args.height) != (1024, 768):
+        W, H = args.width, args.height
+    res = cache_comparison(args, args.cache_frames, W, H)
+    last = res["cached"][-1]
+    paths = W * H * last["frames"]
+    out = base_line(paths / (last["ms"] / 1e3) / 1e6, last["ms"], args, {
+        "workload": res["workload"],
+        "tracer": f"radiance cache ({res['cache']})",
         "paths_per_step": paths,
-        "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation",
+        "reference": res["reference"],
     })
     out["roofline"] = None
     out["cpu_baseline"] = None
-    out["equal_relmse_speedup_vs_plain"] = best["speedup"]
-    out["plain_relmse_times_s"] = round(c, 5)
-    out["plain"] = [dict(spp=s, ms=round(dt * 1e3, 1), relmse=round(e, 5)) for s, dt, e in plain]
-    out["cached"] = cached
+    out["equal_relmse_speedup_vs_plain"] = res["equal_relmse_speedup"]
+    out["plain_relmse_times_s"] = res["plain_relmse_times_s"]
+    out["plain"] = res["plain"]
+    out["cached"] = res["cached"]
     print(json.dumps(out))
 
 
 def bench_city(args):
-    """The ~6M-triangle city, 1920x1080 at 1024 spp, depth 8, RR 5: the plain tracer
-    (the reference image: FP64 accumulation), the batched pipeline (one captured pipeline
-    wave replayed) and the learned radiance cache, each timed; relMSE against
-    the plain 1024-spp render at equal sample counts."""
+    """The ~6M-triangle city, 1920x1080 at 1024 spp, depth 8: the plain tracer at the
+    configuration's 1024 spp (timed: `value`), the batched pipeline (one captured
+    pipeline wave per pool, replayed) on the same render, and the learned radiance
+    cache, each judged by relMSE against a plain FP64 render of --ref-spp samples
+    per pixel (16,384 by default)."""
     import numpy as np
 
     import tinypt_amd
@@ -415,44 +508,67 @@ def bench_city(args):
     print(f"# city ready in {time.perf_counter() - t:.1f} s: {info}", file=sys.stderr, flush=True)
     pt = tinypt_amd.PathTracer("hip")
     kw = dict(max_depth=c["max_depth"], rr_depth=c["rr_depth"])
+    if args.render_only:
+        pt.render(dev, args.spp, seed=1, hdr=True, **kw)
+        print(f"# render {args.spp} spp: {pt.last_stats()}", file=sys.stderr, flush=True)
+        if not args.no_count:
+            pt.render(dev, args.spp, seed=1, hdr=True, count_traffic=True, **kw)
+            print(f"# counting render: {pt.last_stats()}", file=sys.stderr, flush=True)
+        return
     pt.render(dev, 4, seed=1, hdr=True, **kw)  # warm-up
     t = time.perf_counter()
-    ref = pt.render(dev, c["spp"], seed=1, hdr=True, fp64=True, **kw).raw()
-    t_ref = time.perf_counter() - t
-    st = pt.last_stats()
-    print(f"# plain {c['spp']} spp (reference image): {t_ref:.1f} s", file=sys.stderr, flush=True)
+    ref = pt.render(dev, args.ref_spp, seed=777, hdr=True, fp64=True, **kw).raw()
+    ref_s = time.perf_counter() - t
+    print(f"# reference {args.ref_spp} spp: {ref_s:.1f} s", file=sys.stderr, flush=True)
 
     def rel(x):
         return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
-    rows = {}
-    spp = 64
     t = time.perf_counter()
-    img = pt.render(dev, spp, seed=2, hdr=True, **kw).raw()
-    rows["plain_64spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(img), 5))
-    pt.render_batch([dev], [4], [2], hdr=True, **kw)  # autotunes queue capacity and workgroup size
+    img = pt.render(dev, c["spp"], seed=1, hdr=True, **kw).raw()
+    t_plain = time.perf_counter() - t
+    st = pt.last_stats()
+    rows = {"plain_1024spp": dict(ms=round(t_plain * 1e3, 1), relmse=round(rel(img), 6))}
+    print(f"# plain {c['spp']} spp: {t_plain:.2f} s", file=sys.stderr, flush=True)
+    pt.render_batch([dev], [16], [1], hdr=True, **kw)  # autotunes queue capacity and workgroup size
     bst = pt.last_batch_stats()
     cap, blk = bst["queue_capacity"], bst["block_size"]
     t = time.perf_counter()
-    imgs = pt.render_batch([dev], [spp], [2], hdr=True, queue_capacity=cap, block_size=blk, **kw)
-    rows["batch_64spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(imgs[0].raw()), 5),
-                               equals_plain=bool(np.array_equal(imgs[0].raw(), img)),
-                               queue_capacity=cap, block_size=blk, tune_ms=round(bst["tune_ms"], 1))
-    t = time.perf_counter()
-    cimg, cst = tinypt_amd.render_cached(dev, spp, cache_depth=2, train_records=1 << 16, reset=True, seed=2, **kw)
-    rows["cached_64frames"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(cimg), 5),
-                                   mean_bias=round(float(cimg.mean() / ref.mean() - 1.0), 4))
+    imgs = pt.render_batch([dev], [c["spp"]], [1], hdr=True, queue_capacity=cap, block_size=blk, **kw)
+    rows["batch_1024spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1),
+                                 equals_plain=bool(np.array_equal(imgs[0].raw(), img)),
+                                 queue_capacity=cap, block_size=blk, tune_ms=round(bst["tune_ms"], 1))
+    for spp in (64, 256):
+        t = time.perf_counter()
+        p_img = pt.render(dev, spp, seed=2, hdr=True, **kw).raw()
+        rows[f"plain_{spp}spp"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(p_img), 5))
+        t = time.perf_counter()
+        cimg, cst = tinypt_amd.render_cached(dev, spp, cache_depth=2, train_records=1 << 16, reset=True, seed=2,
+                                             weight_seed=5, suffix_len=args.cache_suffix, **kw)
+        rows[f"cached_{spp}frames"] = dict(ms=round((time.perf_counter() - t) * 1e3, 1), relmse=round(rel(cimg), 5),
+                                           mean_bias=round(float(cimg.mean() / ref.mean() - 1.0), 4))
     for k, v in rows.items():
         print(f"# {k}: {v}", file=sys.stderr, flush=True)
-    out = base_line(st["paths"] / t_ref / 1e6, t_ref * 1e3, args, {
-        "workload": f"city_{info['triangles']}tri_{c['width']}x{c['height']}_{c['spp']}spp_depth8_rr5_"
+    # extend roofline on 16-spp renders (two full waves of 8 samples per pixel, as each wave of
+    # the 1024-spp render): co-scheduled timing, counting render, one-stream handle
+    pt.render(dev, 16, seed=1, hdr=True, **kw)
+    ext16 = pt.last_stats()["extend_ms"]
+    pt.render(dev, 16, seed=1, hdr=True, count_traffic=True, **kw)
+    cnt = pt.last_stats()
+    roofline, extra = extend_roofline(pt, host, info, dict(seed=1, hdr=True, **kw), 16, [ext16], cnt,
+                                      CITY_EXTEND_COUNTERS if (c["width"], c["height"]) == (1920, 1080) else None)
+    print(f"# roofline: {roofline}", file=sys.stderr, flush=True)
+    out = base_line(st["paths"] / t_plain / 1e6, t_plain * 1e3, args, {
+        "workload": f"city_{info['triangles']}tri_{c['width']}x{c['height']}_{c['spp']}spp_depth8_"
                     "hdr_sky_cdf_sun_1000_point_lights",
-        "tracer": "plain HIP wavefront (reference image, FP64 accumulation)",
+        "tracer": "plain HIP wavefront",
         "paths_per_step": st["paths"],
+        "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation ({ref_s:.1f} s)",
     })
-    out["roofline"] = None
+    out["roofline"] = roofline
+    out.update(extra)
     out["cpu_baseline"] = None
-    out["at_64spp_vs_reference"] = rows
+    out["vs_reference"] = rows
     print(json.dumps(out))
 
 
