@@ -206,6 +206,20 @@ int tpt_hip_scene_info(const tpt_hip_scene *scene, uint64_t *arena_bytes, uint64
 int tpt_hip_render(tpt_hip_scene *scene, const tpt_render_params *params, float *out_rgb,
                    tpt_render_stats *stats);
 
+/* Render without allocation.  No counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Renders as tpt_hip_render does (the
+ * same image for the same params and seed) with one path pool whose path state,
+ * queues, counters and per-pixel sums live in a fixed device array of the
+ * library (about 130 MB, 2^19 paths and 2^19 pixels; one array per device, held
+ * until the process ends).  Enqueued on `stream` (a hipStream_t, NULL = the null
+ * stream) with no synchronisation, no allocation and no memset: writes H*W*3
+ * floats (row 0 at the bottom) into the DEVICE buffer `out_rgb`.  Rejects scenes
+ * of more than 2^19 pixels, queue_capacity above 2^19 and count_traffic.  All
+ * calls on one device share its array: calls that overlap on a device race
+ * (order them on one stream). */
+int tpt_hip_render_noalloc(tpt_hip_scene *scene, const tpt_render_params *params, float *out_rgb, void *stream);
+
 /* ---- batch: many independent renders in one pipeline ----------------------
  * No counterpart in the reference, which renders one scene per
  * PathTracer::render call (path_tracer.h:26-44); each scene of a batch renders

@@ -124,6 +124,7 @@ C_ABI_SYMBOLS = (
     "tpt_hip_scene_destroy",
     "tpt_hip_scene_info",
     "tpt_hip_render",
+    "tpt_hip_render_noalloc",
     "tpt_hip_trace_closest",
     "tpt_hip_trace_occluded",
     "tpt_hip_last_trace_counters",
@@ -157,6 +158,7 @@ def lib():
     L.tpt_hip_scene_destroy.argtypes = [vp]
     L.tpt_hip_scene_info.argtypes = [vp] + [ctypes.POINTER(u64)] * 5
     L.tpt_hip_render.argtypes = [vp, vp, vp, ctypes.POINTER(_RenderStats)]
+    L.tpt_hip_render_noalloc.argtypes = [vp, ctypes.POINTER(_RenderParams), vp, vp]
     L.tpt_hip_trace_closest.argtypes = [vp, vp, u64, vp, vp, u32, vp]
     L.tpt_hip_trace_occluded.argtypes = [vp, vp, u64, vp, u32, vp]
     L.tpt_hip_last_trace_counters.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -193,6 +195,18 @@ def _check(rc):
         if rc == -1:
             raise ValueError(msg)
         raise RuntimeError(msg)
+
+
+def render_noalloc(scene_hip, out_ptr, spp, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, hdr=False,
+                   fp64=False, queue_capacity=0, block_size=0, stream=0):
+    """tpt_hip_render_noalloc: enqueue a render on `stream` (a hipStream_t as an
+    int, 0 = the null stream) into the device buffer at `out_ptr` (H x W x 3
+    float32, row 0 at the bottom); allocates nothing and does not synchronise."""
+    p = _RenderParams(spp=spp, max_depth=max_depth, rr_depth=rr_depth, shadow_bounded=int(shadow_bounded), seed=seed,
+                      output=1 if hdr else 0, accum_fp64=int(fp64), queue_capacity=queue_capacity,
+                      block_size=block_size)
+    _check(lib().tpt_hip_render_noalloc(scene_hip._hip_ptr(), ctypes.byref(p), ctypes.c_void_p(out_ptr),
+                                        ctypes.c_void_p(stream)))
 
 
 def trace_closest(scene_hip, rays, count_traffic=False):

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <string>
@@ -43,6 +44,7 @@ void launch_cache_apply(const DPaths &, const DCache &, const float4 *, int, hip
 void launch_cache_targets(const DPaths &, const DCache &, float4 *, hipStream_t);
 void launch_frame_accumulate(const DPaths &, int, int, int, hipStream_t);
 void launch_frame_finalize(const DPaths &, int, int, int, float *, hipStream_t);
+void launch_fill_u32(uint32_t *, int, uint32_t, hipStream_t);
 } // namespace tpt
 
 using namespace tpt;
@@ -303,6 +305,58 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     }
 }
 
+// Byte offsets of a path pool of `slots` paths + accumulators / output of `pixels`
+// pixels in one block (every array 256-B aligned).  `with_sid`: also the per-slot
+// scene ids of batch mode; `with_out`: also the finished image.
+struct PathLayout {
+    size_t org, dir, dec, col, rng, key, dep, qa, qb, ss, so, sd, sv, hit, cnt, sf, sdd, pf, pd, out, sid, total;
+};
+constexpr size_t align256c(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr PathLayout path_layout(size_t slots, size_t pixels, bool with_sid, bool with_out) {
+    PathLayout L{};
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        size_t o = off;
+        off = align256c(off + bytes);
+        return o;
+    };
+    L.org = take(slots * 16), L.dir = take(slots * 16), L.dec = take(slots * 16), L.col = take(slots * 16);
+    L.rng = take(slots * 8), L.key = take(slots * 8), L.dep = take(slots * 4);
+    L.qa = take(slots * 4), L.qb = take(slots * 4), L.ss = take(slots * 4);
+    L.so = take(slots * 16), L.sd = take(slots * 16), L.sv = take(slots * 16), L.hit = take(slots * 16);
+    L.cnt = take(kCntWords * 4);
+    L.sf = take(pixels * 12), L.sdd = take(pixels * 24), L.pf = take(pixels * 12), L.pd = take(pixels * 24);
+    L.out = with_out ? take(pixels * 12) : 0;
+    L.sid = with_sid ? take(slots * 4) : 0;
+    L.total = off;
+    return L;
+}
+
+// Points P (and *out) into a block laid out by path_layout.
+void bind_paths(uint8_t *b, const PathLayout &L, bool with_sid, bool with_out, DPaths &P, float **out) {
+    P.sid = with_sid ? (int32_t *)(b + L.sid) : nullptr;
+    P.org = (float4 *)(b + L.org);
+    P.dir = (float4 *)(b + L.dir);
+    P.decay = (float4 *)(b + L.dec);
+    P.color = (float4 *)(b + L.col);
+    P.rng = (uint64_t *)(b + L.rng);
+    P.key = (uint64_t *)(b + L.key);
+    P.depth = (int32_t *)(b + L.dep);
+    P.queue_a = (int32_t *)(b + L.qa);
+    P.queue_b = (int32_t *)(b + L.qb);
+    P.shadow_slot = (int32_t *)(b + L.ss);
+    P.shadow_org = (float4 *)(b + L.so);
+    P.shadow_dir = (float4 *)(b + L.sd);
+    P.shadow_val = (float4 *)(b + L.sv);
+    P.hit = (float4 *)(b + L.hit);
+    P.counters = (uint32_t *)(b + L.cnt);
+    P.sub_f = (float *)(b + L.sf);
+    P.sub_d = (double *)(b + L.sdd);
+    P.pix_f = (float *)(b + L.pf);
+    P.pix_d = (double *)(b + L.pd);
+    if (out) *out = with_out ? (float *)(b + L.out) : nullptr;
+}
+
 // Path pool of `slots` paths + accumulators / output of `pixels` pixels (one
 // allocation).  `with_sid`: also the per-slot scene ids of batch mode.
 void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
@@ -310,48 +364,13 @@ void alloc_paths(PathBuffers &pb, size_t slots, size_t pixels, bool with_sid) {
         HIP_OK(hipFree(pb.mem));
         pb.mem = nullptr;
     }
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        size_t o = off;
-        off = align256(off + bytes);
-        return o;
-    };
-    size_t o_org = take(slots * 16), o_dir = take(slots * 16), o_dec = take(slots * 16), o_col = take(slots * 16);
-    size_t o_rng = take(slots * 8), o_key = take(slots * 8), o_dep = take(slots * 4);
-    size_t o_qa = take(slots * 4), o_qb = take(slots * 4), o_ss = take(slots * 4);
-    size_t o_so = take(slots * 16), o_sd = take(slots * 16), o_sv = take(slots * 16), o_hit = take(slots * 16);
-    size_t o_cnt = take(kCntWords * 4);
-    size_t o_sf = take((size_t)pixels * 12), o_sdd = take((size_t)pixels * 24), o_pf = take((size_t)pixels * 12),
-           o_pd = take((size_t)pixels * 24), o_out = take((size_t)pixels * 12);
-    size_t o_sid = with_sid ? take(slots * 4) : 0;
-    HIP_OK(hipMalloc(&pb.mem, off));
-    pb.bytes = off;
+    const PathLayout L = path_layout(slots, pixels, with_sid, true);
+    HIP_OK(hipMalloc(&pb.mem, L.total));
+    pb.bytes = L.total;
     pb.slots = slots;
     pb.pixels = (int)std::min<size_t>(pixels, (size_t)INT32_MAX);
-    uint8_t *b = (uint8_t *)pb.mem;
-    DPaths &P = pb.P;
-    P.sid = with_sid ? (int32_t *)(b + o_sid) : nullptr;
-    P.org = (float4 *)(b + o_org);
-    P.dir = (float4 *)(b + o_dir);
-    P.decay = (float4 *)(b + o_dec);
-    P.color = (float4 *)(b + o_col);
-    P.rng = (uint64_t *)(b + o_rng);
-    P.key = (uint64_t *)(b + o_key);
-    P.depth = (int32_t *)(b + o_dep);
-    P.queue_a = (int32_t *)(b + o_qa);
-    P.queue_b = (int32_t *)(b + o_qb);
-    P.shadow_slot = (int32_t *)(b + o_ss);
-    P.shadow_org = (float4 *)(b + o_so);
-    P.shadow_dir = (float4 *)(b + o_sd);
-    P.shadow_val = (float4 *)(b + o_sv);
-    P.hit = (float4 *)(b + o_hit);
-    P.counters = (uint32_t *)(b + o_cnt);
-    P.sub_f = (float *)(b + o_sf);
-    P.sub_d = (double *)(b + o_sdd);
-    P.pix_f = (float *)(b + o_pf);
-    P.pix_d = (double *)(b + o_pd);
-    pb.out = (float *)(b + o_out);
-    HIP_OK(hipMemset(P.counters, 0, kCntWords * 4));
+    bind_paths((uint8_t *)pb.mem, L, with_sid, true, pb.P, &pb.out);
+    HIP_OK(hipMemset(pb.P.counters, 0, kCntWords * 4));
 }
 
 void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
@@ -664,6 +683,98 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
                 stats->rays_shadow = sum4[3];
             }
         }
+        return TPT_OK;
+    } catch (const tpt_error &e) {
+        return set_err(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_err(TPT_ERR_RUNTIME, e.what());
+    }
+}
+
+// ---------------------------------------------------------------- render without allocation
+// One path pool of kNoAllocSlots paths and the per-pixel sums of kNoAllocPixels
+// pixels in a fixed device array (`scratch`, one per device, held until the
+// process ends), for callers that cannot allocate.  In host code `scratch` names
+// a host-side stand-in, not the device array: the array's address on a device
+// comes from hipGetSymbolAddress with that device current, once per device.
+namespace {
+constexpr size_t kNoAllocSlots = (size_t)1 << 19;
+constexpr size_t kNoAllocPixels = (size_t)1 << 19;
+constexpr PathLayout kNoAllocLayout = path_layout(kNoAllocSlots, kNoAllocPixels, false, false);
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocLayout.total];
+
+namespace {
+// Device address of `scratch` on `device` (current), looked up on the first call
+// for that device.
+uint8_t *scratch_on(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        throw tpt_error(TPT_ERR_UNSUPPORTED, "tpt_hip_render_noalloc: device index beyond " +
+                                                 std::to_string(kNoAllocMaxDevices));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw tpt_error(TPT_ERR_HIP, "tpt_hip_render_noalloc: no device address for the scratch array");
+        g_scratch_ptr[device] = ptr;
+    }
+    return (uint8_t *)g_scratch_ptr[device];
+}
+} // namespace
+
+extern "C" int tpt_hip_render_noalloc(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb, void *stream) {
+    if (!h || !p || !out_rgb) return set_err(TPT_ERR_INVALID_ARGUMENT, "null argument");
+    if (p->spp <= 0) return set_err(TPT_ERR_INVALID_ARGUMENT, "spp must be positive");
+    if (p->max_depth < 0 || p->max_depth > 1000) return set_err(TPT_ERR_INVALID_ARGUMENT, "max_depth out of range");
+    if (p->count_traffic) return set_err(TPT_ERR_INVALID_ARGUMENT, "tpt_hip_render_noalloc: count_traffic is not supported");
+    const int block = p->block_size > 0 ? p->block_size : 256;
+    if (block % 64 != 0 || block > 1024) return set_err(TPT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 <= 1024");
+    const DScene &S = h->S;
+    const size_t pixels = (size_t)S.cam.width * (size_t)S.cam.height;
+    if (pixels > kNoAllocPixels)
+        return set_err(TPT_ERR_INVALID_ARGUMENT, "tpt_hip_render_noalloc: " + std::to_string(pixels) +
+                                                     " pixels exceed its limit of " + std::to_string(kNoAllocPixels));
+    if (p->queue_capacity < 0 || (size_t)p->queue_capacity > kNoAllocSlots)
+        return set_err(TPT_ERR_INVALID_ARGUMENT, "tpt_hip_render_noalloc: queue_capacity must be in 0 .. " +
+                                                     std::to_string(kNoAllocSlots));
+    try {
+        HIP_OK(hipSetDevice(h->device));
+        uint8_t *base = scratch_on(h->device);
+        hipStream_t st = (hipStream_t)stream;
+        DPaths P{};
+        bind_paths(base, kNoAllocLayout, false, false, P, nullptr);
+        DRender R = make_render(p, p->spp, p->seed, (int)pixels);
+        const size_t cap = p->queue_capacity > 0 ? (size_t)p->queue_capacity : kNoAllocSlots;
+        // samples per pixel per pipeline wave; pixels * k <= max(cap, pixels) <= kNoAllocSlots
+        const int k = (int)std::max<size_t>(1, std::min<size_t>((size_t)R.spp_eff, cap / pixels));
+        R.k = k;
+        // the array keeps the previous call's contents: the queue and refill counters start at zero
+        launch_fill_u32(P.counters, kCntWords, 0u, st);
+        const uint64_t base_mixed = splitmix64(p->seed);
+        int first = 1;
+        for (int wave_base = 0; wave_base < R.spp_eff; wave_base += k) {
+            const int k_active = std::min(k, R.spp_eff - wave_base);
+            const int n = (int)pixels * k_active;
+            launch_generate(S, R, P, wave_base, n, base_mixed, block, st);
+            const int grid = grid_for((size_t)n, block, h->cus);
+            int cur = 0;
+            for (int b = 0; b <= R.max_depth; b++) {
+                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
+                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, false, (uint64_t)n, st);
+                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, st, nullptr);
+                launch_shadow(S, P, grid, block, false, (uint64_t)n, st);
+                cur = 1 - cur;
+            }
+            launch_accumulate(R, P, wave_base, k_active, first, st);
+            first = 0;
+        }
+        launch_finalize(R, P, out_rgb, st);
+        HIP_OK(hipGetLastError());
         return TPT_OK;
     } catch (const tpt_error &e) {
         return set_err(e.code, e.what());

@@ -1218,6 +1218,13 @@ __global__ void k_trace_occluded(DScene S, const float4 *rays, uint64_t n, uint8
     }
 }
 
+// Sets n words to v: clears counters with a kernel (no memset node when a caller
+// captures the work in a graph).
+__global__ void k_fill_u32(uint32_t *p, int n, uint32_t v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ---------------------------------------------------------------- radiance cache (progressive frames)
 __global__ void k_cache_reset(DCache C) {
     if (threadIdx.x < 2) C.counters[threadIdx.x] = 0u;
@@ -1962,6 +1969,9 @@ void launch_batch_finalize(const DBatch &B, const DPaths &P, float *out, uint64_
     hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P, out);
 }
 
+void launch_fill_u32(uint32_t *p, int n, uint32_t v, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
 void launch_cache_reset(const DCache &C, hipStream_t st) { hipLaunchKernelGGL(k_cache_reset, dim3(1), dim3(64), 0, st, C); }
 void launch_cache_apply(const DPaths &P, const DCache &C, const float4 *pred, int n_max, hipStream_t st) {
     hipLaunchKernelGGL(k_cache_apply, dim3((n_max + 255) / 256), dim3(256), 0, st, P, C, pred, n_max);
