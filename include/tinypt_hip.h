@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define TPT_ABI_VERSION 2
+#define TPT_ABI_VERSION 3
 
 /* ---- error codes ------------------------------------------------------- */
 enum {
@@ -124,6 +124,18 @@ typedef struct {
     float radius;    /* point: sphere radius; distant: angle (full cone angle, radians) */
 } tpt_delta_light;
 
+/* Non-delta lights passed to Scene() (cpu/scene.cpp:21-27; ObjectLight and
+ * EnvLight, cpu/light.h:30-43, 113-124).  Each joins the object-light mixture,
+ * in order, after the scene's emissive objects and before the background's own
+ * EnvLight.  TPT_OBJECT_LIGHT_OBJECT: ObjectLight of tpt_scene_desc.objects[object]
+ * (a mesh samples a shape, then a face, uniformly: ObjectGroup::sample,
+ * cpu/object.cpp:352-365); TPT_OBJECT_LIGHT_ENV: EnvLight of the scene's background. */
+enum { TPT_OBJECT_LIGHT_OBJECT = 0, TPT_OBJECT_LIGHT_ENV = 1 };
+typedef struct {
+    int32_t kind;
+    int32_t object;
+} tpt_object_light;
+
 /* ---- camera (cpu/camera.h:14-79) and background (cpu/texture.h:83-109) ---- */
 typedef struct {
     float center[3];
@@ -150,6 +162,8 @@ typedef struct {
     const tpt_material *materials;
     const tpt_image *images;
     const tpt_delta_light *delta_lights;
+    int32_t num_object_lights;
+    const tpt_object_light *object_lights; /* non-delta lights passed to Scene(), in order */
 } tpt_scene_desc;
 
 /* ---- render parameters ---------------------------------------------------- */

@@ -229,8 +229,8 @@ struct MeshData {
 };
 
 struct Light {
-    int kind;   // 0 prim, 1 env
-    int prim;   // prim index
+    int kind;   // 0 prim, 1 env, 2 whole mesh (ObjectLight of a TriangleMesh)
+    int prim;   // prim index (kind 0), top-level object index (kind 2)
 };
 struct DeltaLight {
     int type;
@@ -602,15 +602,38 @@ struct oracle_scene {
         V2 uv = dir2uv(mat_apply(bg_rot, dir));
         return env_dist.pdf(uv) * (0.25f * INV_PI_F);
     }
+    // ObjectLight of a mesh: ObjectGroup::sample / pdf (object.cpp:352-365) over the
+    // mesh's shapes, each an ObjectGroup over its faces (shapes without faces left out)
+    V3 mesh_sample(const TopObj &t, V3 pos, RandEngine &rng) const {
+        std::vector<const std::vector<int> *> shapes;
+        for (const auto &f : t.shape_faces)
+            if (!f.empty()) shapes.push_back(&f);
+        const std::vector<int> &faces = *shapes[rng.rand_int((int)shapes.size())];
+        return prim_sample(prims[faces[rng.rand_int((int)faces.size())]], pos, rng);
+    }
+    float mesh_pdf(const TopObj &t, const Ray &r) const {
+        float val = 0;
+        int n = 0;
+        for (const auto &faces : t.shape_faces) {
+            if (faces.empty()) continue;
+            float sv = 0;
+            for (int pi : faces) sv += prim_pdf(prims[pi], r);
+            val += sv / (float)faces.size();
+            n++;
+        }
+        return val / (float)n;
+    }
     V3 lights_sample(V3 pos, RandEngine &rng) const { // LightGroup::sample (light.cpp:35-39)
         const Light &l = obj_lights[rng.rand_int((int)obj_lights.size())];
         if (l.kind == 1) return env_sample(rng);
+        if (l.kind == 2) return mesh_sample(top[l.prim], pos, rng);
         return prim_sample(prims[l.prim], pos, rng);
     }
     float lights_pdf(V3 pos, V3 dir) const { // LightGroup::pdf (light.cpp:41-48)
         float val = 0;
         Ray r{pos, dir};
-        for (const Light &l : obj_lights) val += (l.kind == 1) ? env_pdf(dir) : prim_pdf(prims[l.prim], r);
+        for (const Light &l : obj_lights)
+            val += (l.kind == 1) ? env_pdf(dir) : (l.kind == 2) ? mesh_pdf(top[l.prim], r) : prim_pdf(prims[l.prim], r);
         return val / (float)obj_lights.size();
     }
     // DistantLight::sample / PointLight::sample (light.cpp:11-27); tmax = distance bound
@@ -1146,6 +1169,19 @@ static oracle_scene *build_scene(const tpt_scene_desc *d) {
             }
         }
     }
+    // non-delta lights passed to Scene() join the mixture after the emissive objects (scene.cpp:21-27)
+    for (int i = 0; i < d->num_object_lights; i++) {
+        const tpt_object_light &ol = d->object_lights[i];
+        if (ol.kind == TPT_OBJECT_LIGHT_ENV) {
+            s->obj_lights.push_back({1, -1});
+            continue;
+        }
+        if (ol.kind != TPT_OBJECT_LIGHT_OBJECT || ol.object < 0 || ol.object >= (int)s->top.size())
+            throw std::invalid_argument("invalid object light");
+        const TopObj &t = s->top[ol.object];
+        if (t.type == TPT_OBJ_MESH) s->obj_lights.push_back({2, ol.object});
+        else s->obj_lights.push_back({0, t.prim});
+    }
     for (int i = 0; i < d->num_delta_lights; i++) {
         const tpt_delta_light &l = d->delta_lights[i];
         DeltaLight g;
@@ -1243,7 +1279,8 @@ int oracle_object_lights(const oracle_scene *s, int32_t *kinds, int32_t *ids, in
     int n = (int)s->obj_lights.size();
     for (int i = 0; i < n && i < cap; i++) {
         kinds[i] = s->obj_lights[i].kind;
-        ids[i] = s->obj_lights[i].kind == 0 ? s->prims[s->obj_lights[i].prim].gid : -1;
+        const Light &l = s->obj_lights[i];
+        ids[i] = l.kind == 0 ? s->prims[l.prim].gid : l.kind == 2 ? l.prim : -1;
     }
     return n;
 }
@@ -1504,11 +1541,15 @@ int oracle_light_sample(const oracle_scene *s, int32_t li, const float *pos3, ui
     RandEngine rng(seed);
     V3 pos{pos3[0], pos3[1], pos3[2]};
     const Light &l = s->obj_lights[li];
-    V3 d = (l.kind == 1) ? s->env_sample(rng) : s->prim_sample(s->prims[l.prim], pos, rng);
+    V3 d = (l.kind == 1)   ? s->env_sample(rng)
+           : (l.kind == 2) ? s->mesh_sample(s->top[l.prim], pos, rng)
+                           : s->prim_sample(s->prims[l.prim], pos, rng);
     dir3[0] = d.x;
     dir3[1] = d.y;
     dir3[2] = d.z;
-    *pdf = (l.kind == 1) ? s->env_pdf(d) : s->prim_pdf(s->prims[l.prim], Ray{pos, d});
+    *pdf = (l.kind == 1)   ? s->env_pdf(d)
+           : (l.kind == 2) ? s->mesh_pdf(s->top[l.prim], Ray{pos, d})
+                           : s->prim_pdf(s->prims[l.prim], Ray{pos, d});
     return 0;
 }
 

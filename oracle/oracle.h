@@ -26,7 +26,8 @@ oracle_scene *oracle_scene_create(const tpt_scene_desc *desc);
 void oracle_scene_destroy(oracle_scene *s);
 
 /* Object-light list in reference order (cpu/scene.cpp:6-36): kind 0 = primitive
- * (id = global primitive id), kind 1 = environment light.  Returns count. */
+ * (id = global primitive id), kind 1 = environment light, kind 2 = ObjectLight of a
+ * whole mesh (id = its index in tpt_scene_desc.objects).  Returns count. */
 int oracle_object_lights(const oracle_scene *s, int32_t *kinds, int32_t *ids, int32_t cap);
 
 /* Closest hit, geometric only.  mode 0: reference structure (linear top level,

@@ -88,7 +88,7 @@ def test_fp64_accumulation_matches(hip):
     check(g, o, o2, 0.90)
 
 
-@pytest.mark.parametrize("variant", ["reference", "importance_bounded_rr", "thin_lens"])
+@pytest.mark.parametrize("variant", ["reference", "importance_bounded_rr", "thin_lens", "object_lights"])
 def test_feature_zoo(hip, variant):
     """Textures (f32 + RGBA8), alpha cut-out, bump, vertex normals / uvs, every BxDF kind,
     sphere / rect / circle / mesh lights, point + distant lights, env map, thin lens."""
@@ -100,8 +100,10 @@ def test_feature_zoo(hip, variant):
     elif variant == "importance_bounded_rr":
         scene = zoo.feature_zoo(importance=True)
         kw.update(max_depth=10, rr_depth=3, shadow_bounded=True)
-    else:
+    elif variant == "thin_lens":
         scene = zoo.feature_zoo(aperture=0.3)
+    else:  # ObjectLight (sphere, mesh, quad mesh) and EnvLight passed to Scene() join the light mixture
+        scene = zoo.feature_zoo(importance=True, object_lights=True)
     g, o, o2 = render_both(hip, scene, 16, seed=11, hdr=True, **kw)
     check(g, o, o2, 0.75)
 

@@ -27,7 +27,7 @@ def test_c_abi_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(T.C_ABI_SYMBOLS)
-    assert T.lib().tpt_hip_abi_version() == T.ABI_VERSION == 2
+    assert T.lib().tpt_hip_abi_version() == T.ABI_VERSION == 3
 
 
 def test_device_strings():
@@ -123,3 +123,70 @@ def test_mesh_transform_is_baked():
     rays = np.array([[0.2, 0.2, 10, 1e-3, 0, 0, -1, 1e20]], dtype=np.float32)
     p, t = o.trace_closest(rays)
     assert p[0] == 0 and t[0] == np.float32(5.0)
+
+
+def test_object_and_env_lights_join_the_mixture_in_reference_order():
+    """cpu/scene.cpp:9-30: emissive objects in DFS order, then the non-delta lights
+    passed to Scene() in their order (delta lights go to their own group), then the
+    background's EnvLight when it is a map."""
+    import oracle_py as O
+    import zoo
+
+    scene = zoo.feature_zoo(16, 12, importance=True, object_lights=True)
+    base = zoo.feature_zoo(16, 12, importance=True)
+    got = O.OracleScene(scene).object_lights()
+    dfs = O.OracleScene(base).object_lights()
+    assert dfs[-1] == (1, -1) and all(k == 0 for k, _ in dfs[:-1])
+    # ObjectLight(sphere) -> its primitive; ObjectLight(mesh) -> kind 2 with the object index
+    # (objects: floor, back, 5 spheres, icosphere mesh 7, quad mesh 8, circle 9); EnvLight(bg)
+    sphere_gid = 2
+    assert got == dfs[:-1] + [(0, sphere_gid), (2, 7), (1, -1), (2, 8), (1, -1)]
+
+
+def test_object_and_env_light_errors():
+    import zoo
+
+    s = T.Sphere()
+    outside = T.Sphere()
+    cam = T.Camera()
+    with pytest.raises(ValueError, match="ObjectLight"):
+        T.Scene(cam, [s], [T.ObjectLight(outside)], None)
+    env = T.EnvTexture(T.RGBTexture([1.0, 1.0, 1.0]))
+    with pytest.raises(ValueError, match="EnvLight"):
+        T.Scene(cam, [s], [T.EnvLight(env)], None)
+    with pytest.raises(ValueError, match="EnvLight"):
+        T.Scene(cam, [s], [T.EnvLight()], env)
+    T.Scene(cam, [s], [T.ObjectLight(s), T.EnvLight(env)], env)  # both accepted
+
+
+def test_mesh_object_light_pdf_is_its_sampling_density():
+    """ObjectLight of a two-shape mesh (a 2 x 3 rectangle split into two triangles,
+    one per shape): shape-uniform then face-uniform sampling, pdf = mean over
+    shapes of the mean face pdf (cpu/object.cpp:352-365).  If the pdf is the
+    sampler's density, E[1 / pdf] over its samples is the solid angle the mesh
+    subtends: 4 atan(ab / (2h sqrt(4h^2 + a^2 + b^2))) for a centred a x b
+    rectangle at distance h."""
+    import math
+
+    import oracle_py as O
+
+    a, b, h = 2.0, 3.0, 1.5
+    v = np.array([[-a / 2, -b / 2, h], [a / 2, -b / 2, h], [a / 2, b / 2, h], [-a / 2, b / 2, h]], np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    faces9 = np.repeat(f, 3, axis=1).astype(np.int32)
+    faces9[:, [1, 2, 4, 5, 7, 8]] = -1
+    m = T.Material(T.Lambertian(T.RGBTexture([0.5, 0.5, 0.5])))
+    mesh = T.TriangleMesh.from_arrays(v, faces9, [m], np.zeros(2, np.int32), None, None, [0, 1, 2])
+    cam = T.Camera()
+    o = O.OracleScene(T.Scene(cam, [mesh], [T.ObjectLight(mesh)], None))
+    assert o.object_lights() == [(2, 0)]
+    pos = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    p = ctypes.c_float()
+    inv = []
+    for seed in range(20000):
+        assert O.lib().oracle_light_sample(o.h, 0, pos.ctypes.data, seed, d.ctypes.data, ctypes.byref(p)) == 0
+        assert p.value > 0
+        inv.append(1.0 / p.value)
+    want = 4 * math.atan(a * b / (2 * h * math.sqrt(4 * h * h + a * a + b * b)))
+    assert abs(np.mean(inv) / want - 1) < 0.02, (np.mean(inv), want)

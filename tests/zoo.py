@@ -33,7 +33,10 @@ def checker(w, h, c0, c1, n=8):
     return (np.array(c0, np.float32) * (1 - m) + np.array(c1, np.float32) * m).astype(np.float32)
 
 
-def feature_zoo(width=96, height=72, env=True, importance=False, aperture=0.0):
+def feature_zoo(width=96, height=72, env=True, importance=False, aperture=0.0, object_lights=False):
+    """object_lights: also pass ObjectLight of a glossy sphere, of the icosphere mesh
+    and of the emissive quad, and EnvLight of the background, between the delta
+    lights (they join the object-light mixture, cpu/scene.cpp:21-27)."""
     rng = np.random.default_rng(7)
     objs = []
     # walls: rectangles, one with an f32 checker texture
@@ -123,4 +126,9 @@ def feature_zoo(width=96, height=72, env=True, importance=False, aperture=0.0):
     cam.fov = math.radians(60.0)
     if aperture > 0:
         cam.set_lens(aperture, 9.0)
-    return T.Scene(cam, objs, [p1, p2, sun], bg)
+    lights = [p1, p2, sun]
+    if object_lights:
+        lights = [p1, T.ObjectLight(objs[2]), p2, T.ObjectLight(mesh), sun, T.ObjectLight(quad)]
+        if bg is not None:
+            lights.insert(4, T.EnvLight(bg))
+    return T.Scene(cam, objs, lights, bg)

@@ -32,6 +32,7 @@ from ._C import (  # noqa: E402,F401
     Circle,
     Dielectric,
     DistantLight,
+    EnvLight,
     EnvTexture,
     Glossy,
     Image,
@@ -41,6 +42,7 @@ from ._C import (  # noqa: E402,F401
     Metal,
     MixBXDF,
     Object3d,
+    ObjectLight,
     PathTracer,
     PointLight,
     Rectangle,
@@ -141,7 +143,7 @@ C_ABI_SYMBOLS = (
 )
 
 
-ABI_VERSION = 2  # TPT_ABI_VERSION of include/tinypt_hip.h (the ctypes structures above follow it)
+ABI_VERSION = 3  # TPT_ABI_VERSION of include/tinypt_hip.h (the ctypes structures above follow it)
 
 
 def lib():

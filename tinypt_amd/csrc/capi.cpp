@@ -201,6 +201,9 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     size_t o_ec = put(blob, H.env_cond);
     size_t o_gm = put(blob, H.env_gmarg);
     size_t o_gc = put(blob, H.env_gcond);
+    std::vector<int2> ls = H.light_shapes;
+    if (ls.empty()) ls.resize(1);
+    size_t o_ls = put(blob, ls);
     size_t total = align256(blob.size());
     blob.resize(total);
     HIP_OK(hipMalloc(&h->arena, total));
@@ -226,6 +229,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.env_cond = (const float *)(base + o_ec);
     S.env_gmarg = (const int32_t *)(base + o_gm);
     S.env_gcond = (const int32_t *)(base + o_gc);
+    S.light_shapes = (const int2 *)(base + o_ls);
     S.layout = H.layout;
     const char *pers = std::getenv("TPT_PERSISTENT");
     // traversal kernels (room 1024x768x256, Mpaths/s):
@@ -991,7 +995,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             to_offset(T.normals), to_offset(T.texcoords), to_offset(T.gid_local), to_offset(T.analytic);
             to_offset(T.materials), to_offset(T.images), to_offset(T.texels), to_offset(T.delta);
             to_offset(T.lights), to_offset(T.env_marg), to_offset(T.env_cond), to_offset(T.env_gmarg);
-            to_offset(T.env_gcond);
+            to_offset(T.env_gcond), to_offset(T.light_shapes);
             table[i].lds_nodes = 0; // no LDS node cache: wavefronts of one workgroup trace different scenes
             int px = table[i].cam.width * table[i].cam.height;
             rend[i] = make_render(p, items[i].spp, items[i].seed, px);

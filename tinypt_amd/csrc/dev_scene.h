@@ -234,10 +234,28 @@ __device__ __forceinline__ float tri_pdf(const DScene &S, int tri, V3 org, V3 di
     fn = nrm(fn);
     return plane_pdf(t, area, fn, dir);
 }
+// ObjectLight of a whole mesh: a shape, then a face of it, uniformly, then the
+// face's sample; pdf = mean over shapes of the mean over their faces
+// (ObjectGroup::sample / pdf, cpu/object.cpp:352-365)
+__device__ __forceinline__ V3 mesh_light_sample(const DScene &S, const DLight &l, V3 pos, Rng &rng) {
+    const int2 r = S.light_shapes[l.local + rng.rand_int(l.gid)];
+    return tri_sample(S, r.x + rng.rand_int(r.y - r.x), pos, rng);
+}
+__device__ __forceinline__ float mesh_light_pdf(const DScene &S, const DLight &l, V3 pos, V3 dir) {
+    float val = 0.f;
+    for (int s = 0; s < l.gid; s++) {
+        const int2 r = load_invariant(S.light_shapes + l.local + s);
+        float sv = 0.f;
+        for (int tri = r.x; tri < r.y; tri++) sv += tri_pdf(S, tri, pos, dir);
+        val += sv / (float)(r.y - r.x);
+    }
+    return val / (float)l.gid;
+}
 __device__ __forceinline__ V3 lights_sample(const DScene &S, V3 pos, Rng &rng) { // LightGroup::sample
     const DLight l = S.lights[rng.rand_int(S.num_lights) ];
     if (l.kind == L_ENV) return env_sample(S, rng);
     if (l.kind == L_ANALYTIC) return analytic_sample(S.analytic[l.local], pos, rng);
+    if (l.kind == L_MESH) return mesh_light_sample(S, l, pos, rng);
     return tri_sample(S, l.local, pos, rng);
 }
 __device__ __forceinline__ float lights_pdf(const DScene &S, V3 pos, V3 dir) { // LightGroup::pdf
@@ -247,6 +265,7 @@ __device__ __forceinline__ float lights_pdf(const DScene &S, V3 pos, V3 dir) { /
         float p;
         if (l.kind == L_ENV) p = env_pdf(S, dir);
         else if (l.kind == L_ANALYTIC) p = analytic_pdf(load_invariant(S.analytic + l.local), pos, dir);
+        else if (l.kind == L_MESH) p = mesh_light_pdf(S, l, pos, dir);
         else p = tri_pdf(S, l.local, pos, dir);
         val += p;
     }

@@ -101,7 +101,9 @@ struct DDelta {
     float angular_radius;
 };
 
-enum { L_ANALYTIC = 0, L_TRIANGLE = 1, L_ENV = 2 };
+// L_MESH: ObjectLight of a whole mesh: `local` = first of its `gid` shapes in
+// DScene::light_shapes (tri_id ranges), sampled shape-uniform then face-uniform
+enum { L_ANALYTIC = 0, L_TRIANGLE = 1, L_ENV = 2, L_MESH = 3 };
 struct DLight {
     int32_t kind;
     int32_t local; // analytic index / triangle id
@@ -145,6 +147,7 @@ struct DScene {
     // a table is upper_bound(cdf, k / G) for k < G, and count + 1 for k = G
     const int32_t *env_gmarg;
     const int32_t *env_gcond;
+    const int2 *light_shapes; // [first, end) tri_id range of each shape of the L_MESH lights
     int32_t num_nodes;
     int32_t root; // child reference of the root (inner 0, or a leaf when the BVH is one leaf), -1 if empty
     int32_t num_tris;

@@ -1310,6 +1310,7 @@ __device__ __forceinline__ DScene scene_at(const DBatch &B, int s) {
     S.env_cond = rebase(B.base, S.env_cond);
     S.env_gmarg = rebase(B.base, S.env_gmarg);
     S.env_gcond = rebase(B.base, S.env_gcond);
+    S.light_shapes = rebase(B.base, S.light_shapes);
     return S;
 }
 

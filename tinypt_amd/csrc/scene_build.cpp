@@ -663,6 +663,30 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
         }
     }
 
+    // ---- non-delta lights passed to Scene(), after the emissive objects (cpu/scene.cpp:21-27)
+    H.light_shapes.clear();
+    for (int i = 0; i < d.num_object_lights; i++) {
+        if (!d.object_lights) fail(TPT_ERR_INVALID_ARGUMENT, "object_lights is null");
+        const tpt_object_light &ol = d.object_lights[i];
+        if (ol.kind == TPT_OBJECT_LIGHT_ENV) {
+            H.lights.push_back(DLight{L_ENV, -1, -1, 0});
+            continue;
+        }
+        if (ol.kind != TPT_OBJECT_LIGHT_OBJECT) fail(TPT_ERR_UNSUPPORTED, "unsupported object light kind");
+        if (ol.object < 0 || ol.object >= (int)top.size()) fail(TPT_ERR_INVALID_ARGUMENT, "object light index out of range");
+        const TopRec &t = top[ol.object];
+        if (t.type != TPT_OBJ_MESH) {
+            H.lights.push_back(DLight{L_ANALYTIC, t.analytic, H.analytic[t.analytic].gid, 0});
+            continue;
+        }
+        const int first = (int)H.light_shapes.size();
+        for (size_t s = 0; s < t.shape_first.size(); s++)
+            if (t.shape_end[s] > t.shape_first[s]) H.light_shapes.push_back(int2{t.shape_first[s], t.shape_end[s]});
+        const int ns = (int)H.light_shapes.size() - first;
+        if (ns == 0) fail(TPT_ERR_INVALID_ARGUMENT, "ObjectLight of a mesh without faces");
+        H.lights.push_back(DLight{L_MESH, first, ns, 0});
+    }
+
     // ---- delta lights (cpu/light.h:45-111)
     for (int i = 0; i < d.num_delta_lights; i++) {
         const tpt_delta_light &l = d.delta_lights[i];

@@ -32,6 +32,7 @@ struct HostScene {
     std::vector<DDelta> delta;
     std::vector<DLight> lights;
     std::vector<float> env_marg, env_cond;
+    std::vector<int2> light_shapes;                 // shapes of mesh ObjectLights (dev_types.h)
     std::vector<int32_t> env_gmarg, env_gcond; // guide tables of the CDFs (dev_types.h)
     int32_t env_w = 1, env_h = 1, env_gw = 1, env_gh = 1;
     int32_t bg_map = 0;

@@ -118,6 +118,7 @@ struct FlatScene {
     std::vector<tpt_material> materials;
     std::vector<tpt_image> images;
     std::vector<tpt_delta_light> delta;
+    std::vector<tpt_object_light> object_lights;
     std::vector<std::shared_ptr<Image>> image_refs;
     std::map<const Image *, int> image_index;
     struct MeshArrays {
@@ -296,6 +297,20 @@ static std::shared_ptr<FlatScene> flatten(const Camera &cam, const std::vector<s
     }
     for (const auto &l : lights) {
         TINYPT_CHECK_EX(l != nullptr, std::invalid_argument, "null light");
+        // non-delta lights join the object-light mixture (cpu/scene.cpp:21-27)
+        if (auto ol = dynamic_cast<const ObjectLight *>(l.get())) {
+            auto it = std::find(objects.begin(), objects.end(), ol->object);
+            TINYPT_CHECK_EX(ol->object != nullptr && it != objects.end(), std::invalid_argument,
+                            "ObjectLight: its object must be one of the scene's objects");
+            F->object_lights.push_back({TPT_OBJECT_LIGHT_OBJECT, (int32_t)(it - objects.begin())});
+            continue;
+        }
+        if (auto el = dynamic_cast<const EnvLight *>(l.get())) {
+            TINYPT_CHECK_EX(el->env != nullptr && el->env == bg, std::invalid_argument,
+                            "EnvLight: its environment must be the scene's background");
+            F->object_lights.push_back({TPT_OBJECT_LIGHT_ENV, -1});
+            continue;
+        }
         tpt_delta_light d{};
         if (auto dl = dynamic_cast<const DistantLight *>(l.get())) {
             d.type = TPT_LIGHT_DISTANT;
@@ -337,6 +352,8 @@ static std::shared_ptr<FlatScene> flatten(const Camera &cam, const std::vector<s
     D.materials = F->materials.data();
     D.images = F->images.data();
     D.delta_lights = F->delta.data();
+    D.num_object_lights = (int32_t)F->object_lights.size();
+    D.object_lights = F->object_lights.data();
     return F;
 }
 

@@ -319,6 +319,11 @@ PYBIND11_MODULE(_C, m) {
         .def_readwrite("color", &PointLight::color)
         .def_readwrite("power", &PointLight::power)
         .def_readwrite("radius", &PointLight::radius);
+    py::class_<ObjectLight, Light, std::shared_ptr<ObjectLight>>(m, "ObjectLight")
+        .def(py::init<std::shared_ptr<Object>>(), py::arg("object"));
+    py::class_<EnvLight, Light, std::shared_ptr<EnvLight>>(m, "EnvLight")
+        .def(py::init())
+        .def(py::init<std::shared_ptr<EnvTexture>>(), py::arg("env"));
 
     // bxdf (pybind.cpp:155-158)
     py::class_<BXDF, std::shared_ptr<BXDF>>(m, "BXDF");

@@ -351,6 +351,27 @@ class PointLight : public Light {
     float radius = 0;
 };
 
+// ObjectLight (cpu/light.h:30-43): importance sampling towards an object; passed
+// in Scene()'s lights, it joins the object-light mixture after the scene's
+// emissive objects.  Its object must be one of the scene's objects.
+class ObjectLight : public Light {
+  public:
+    ObjectLight() = default;
+    explicit ObjectLight(std::shared_ptr<Object> obj) : object(std::move(obj)) {}
+    bool is_delta() const override { return false; }
+    std::shared_ptr<Object> object;
+};
+// EnvLight (cpu/light.h:113-124): importance sampling of the environment; passed
+// in Scene()'s lights, it joins the mixture as ObjectLight does.  Its environment
+// must be the scene's background.
+class EnvLight : public Light {
+  public:
+    EnvLight() = default;
+    explicit EnvLight(std::shared_ptr<EnvTexture> e) : env(std::move(e)) {}
+    bool is_delta() const override { return false; }
+    std::shared_ptr<EnvTexture> env;
+};
+
 // ---------------------------------------------------------------- camera (cpu/camera.h)
 class Camera {
   public:
