@@ -392,6 +392,19 @@ def bench_batch(args):
     out["cpu_baseline"] = None
     out["scene_completion_ms"] = {"min": round(done[0], 1), "median": round(done[len(done) // 2], 1),
                                   "p90": round(done[int(0.9 * (len(done) - 1))], 1), "max": round(done[-1], 1)}
+    # the same batch with the capacity tuned up to 2^24 slots (above the configuration's
+    # 2^18..2^22): larger pipeline waves amortise each wave's fixed launches
+    os.environ["TPT_BATCH_MAX_CAP"] = "24"
+    try:
+        pt.render_batch(devs, spps, seeds, **kw)
+        st24 = pt.last_batch_stats()
+        t = time.perf_counter()
+        pt.render_batch(devs, spps, seeds, queue_capacity=st24["queue_capacity"], block_size=st24["block_size"], **kw)
+        ms24 = (time.perf_counter() - t) * 1e3
+    finally:
+        os.environ.pop("TPT_BATCH_MAX_CAP")
+    out["tuned_up_to_2^24"] = {"mpaths_per_s": round(paths / (ms24 / 1e3) / 1e6, 2), "ms": round(ms24, 1),
+                               "queue_capacity": st24["queue_capacity"], "block_size": st24["block_size"]}
     if not args.no_seq:
         t = time.perf_counter()
         for d, n, sd in zip(devs, spps, seeds):

@@ -260,8 +260,10 @@ typedef struct {
  * previous wave's, so the images are bit-identical to single-scene renders.
  * `params` gives the shared settings (max_depth, rr_depth, shadow_bounded, output,
  * accum_fp64); its spp and seed are ignored (per scene in `items`).
- * params->queue_capacity == 0 selects the capacity by autotuning over 2^18..2^26
- * slots, params->block_size == 0 the workgroup size over 64..512.  out_rgb[i]
+ * params->queue_capacity == 0 selects the capacity by autotuning over 2^18..2^22
+ * slots (TPT_BATCH_MAX_CAP=23..26 raises the top), params->block_size == 0 the
+ * workgroup size over 64..512.  The path pools stay allocated on the first
+ * (fewest-paths) scene's handle for later batches.  out_rgb[i]
  * receives scene i's image (H*W*3 floats, row 0 at the bottom); `scene_ms` (may be
  * NULL) receives n completion times in ms since the batch started; `stats` may be
  * NULL. */

@@ -85,3 +85,19 @@ def test_batch_pools_equal_single_renders(hip, batch, monkeypatch, streams):
     imgs = hip.PathTracer("hip").render_batch(devs, spps, seeds, queue_capacity=4096, block_size=64, **kw)
     for i, (a, b) in enumerate(zip(imgs, ref)):
         assert np.array_equal(a.raw(), b), f"scene {i} differs"
+
+
+def test_batch_pools_reused_across_calls(hip, batch):
+    # the path pools stay on the first (fewest-paths) scene's handle: a smaller batch
+    # after a larger one, then the larger again, reuse them and still give the
+    # single renders' images
+    hosts, devs, spps, seeds = batch
+    kw = dict(max_depth=5, rr_depth=3, hdr=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    pt = hip.PathTracer("hip")
+    order = [(list(range(5)), 1 << 16), ([3, 0], 1 << 12), (list(range(5)), 1 << 14)]
+    for idx, cap in order:
+        imgs = pt.render_batch([devs[i] for i in idx], [spps[i] for i in idx], [seeds[i] for i in idx],
+                               queue_capacity=cap, block_size=128, **kw)
+        for i, img in zip(idx, imgs):
+            assert np.array_equal(img.raw(), ref[i]), f"scene {i} differs (batch {idx}, capacity {cap})"

@@ -132,6 +132,8 @@ struct tpt_hip_scene {
     hipStream_t stream2 = nullptr;      // stream of lane 1 (several-stream renders), default priority
     hipStream_t streamx[2] = {nullptr, nullptr}; // streams of lanes 2, 3
     PathBuffers pbx[2];                 // path pools of lanes 2, 3, pb's accumulators
+    PathBuffers batch_pb[4];            // path pools of batches whose first (shortest) scene this is,
+                                        // kept across calls (with the per-slot scene ids)
     int streams = 3;                    // plain renders alternate pipeline waves, cached and progressive
                                         // renders frames, over `streams` path pools on as many streams;
                                         // batches use up to three pools
@@ -150,6 +152,8 @@ struct tpt_hip_scene {
         if (pb.mem) (void)hipFree(pb.mem);
         if (pb2.mem) (void)hipFree(pb2.mem);
         for (PathBuffers &x : pbx)
+            if (x.mem) (void)hipFree(x.mem);
+        for (PathBuffers &x : batch_pb)
             if (x.mem) (void)hipFree(x.mem);
         if (stream2) (void)hipStreamDestroy(stream2);
         for (hipStream_t x : streamx)
@@ -801,14 +805,23 @@ namespace {
 #endif
 constexpr int kBatchPools = TPT_BATCH_POOLS;
 
+// Batch path pool k of the handle `h` with at least `slots` paths and sums for
+// `pixels` pixels; kept across batch calls, reallocated only when too small.
+void ensure_batch_pool(tpt_hip_scene *h, int k, size_t slots, uint64_t pixels) {
+    PathBuffers &x = h->batch_pb[k];
+    if (x.mem && x.slots >= slots && (uint64_t)x.pixels >= pixels) return;
+    if (x.mem) sync_lanes(h); // a previous batch may still run on the handle's streams
+    alloc_paths(x, slots, (size_t)pixels, true);
+}
+
 struct BatchRun {
     static constexpr int kMaxPools = 4;
     void *meta = nullptr;                 // scene / sampler tables, pool 0's segments and wave start
-    PathBuffers pb;                       // pool 0, holding the per-pixel sums
+    PathBuffers *pbs[kMaxPools] = {};     // path pools (owned by the first scene's handle); pool 0
+                                          // holds the per-pixel sums the others add into
     DBatch B{};                           // pool 0's view of the batch
     hipStream_t st = nullptr;             // pool 0's stream
     void *xmeta[kMaxPools - 1] = {};      // pools 1..: segment table and wave start
-    PathBuffers xpb[kMaxPools - 1];       // pools 1..: path state, adding into pool 0's sums
     DBatch xB[kMaxPools - 1]{};
     hipStream_t xst[kMaxPools - 1] = {};
     uint64_t total_paths = 0, total_pixels = 0;
@@ -820,24 +833,21 @@ struct BatchRun {
     int max_lds = 65536;
     int pools = 1;
     DBatch &batch(int k) { return k ? xB[k - 1] : B; }
-    PathBuffers &pool(int k) { return k ? xpb[k - 1] : pb; }
+    PathBuffers &pool(int k) { return *pbs[k]; }
     hipStream_t stream(int k) const { return k ? xst[k - 1] : st; }
     ~BatchRun() {
         for (int k = 0; k < pools; k++) // nothing queued may still use the buffers
             if (stream(k)) (void)hipStreamSynchronize(stream(k));
-        if (pb.mem) (void)hipFree(pb.mem);
         if (meta) (void)hipFree(meta);
-        for (int k = 0; k < kMaxPools - 1; k++) {
-            if (xpb[k].mem) (void)hipFree(xpb[k].mem);
+        for (int k = 0; k < kMaxPools - 1; k++)
             if (xmeta[k]) (void)hipFree(xmeta[k]);
-        }
     }
-    // one more pool: its own path state, queues, segment table and wave start
-    void add_pool(size_t slots, hipStream_t stream) {
+    // one more pool: its own path state and queues (`x`, adding into pool 0's
+    // per-pixel sums), segment table and wave start
+    void add_pool(PathBuffers *x, hipStream_t stream) {
         const int k = pools;
-        PathBuffers &x = xpb[k - 1];
-        alloc_paths(x, slots, (size_t)total_pixels, true);
-        share_sums(x, pb);
+        pbs[k] = x;
+        share_sums(*x, *pbs[0]);
         size_t seg_bytes = align256(4 * (size_t)kSegFields * (size_t)B.seg_stride);
         HIP_OK(hipMalloc(&xmeta[k - 1], seg_bytes + 16));
         HIP_OK(hipMemset(xmeta[k - 1], 0, seg_bytes + 16));
@@ -872,7 +882,7 @@ struct BatchRun {
             uint64_t s0[2] = {start + (uint64_t)k * (uint64_t)cap, total_paths};
             HIP_OK(hipMemcpy(batch(k).state, s0, sizeof(s0), hipMemcpyHostToDevice));
         }
-        launch_batch_zero(B, pb.P, fp64, total_pixels, st);
+        launch_batch_zero(B, pbs[0]->P, fp64, total_pixels, st);
     }
     int waves(int cap) const { return (int)((total_paths + (uint64_t)cap - 1) / (uint64_t)cap); }
     // enqueue waves [0, W): wave w's pipeline graph on pool w % pools's stream, then
@@ -1034,26 +1044,31 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.n = n;
         B.seg_stride = seg_stride_for(n);
         B.pools = 1;
-        // candidate capacities: 2^18, 2^20, .. 2^26 path slots, never (much) more than
-        // the batch needs (bigger waves amortise each wave's fixed cost: 64 mixed
-        // rooms run at 472 / 548 / 577 Mpaths/s with 2^22 / 2^24 / 2^26)
+        // candidate capacities: 2^18, 2^20, 2^21, 2^22 path slots (the batch
+        // configuration's range; TPT_BATCH_MAX_CAP raises the top to 2^23 .. 2^26),
+        // never (much) more than the batch needs
         auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
         std::vector<int> caps;
         if (p->queue_capacity > 0) caps.push_back(round_cap((uint64_t)p->queue_capacity));
-        else
-            for (int e = 18; e <= 26; e += 2) {
+        else {
+            const int top = env_int("TPT_BATCH_MAX_CAP", 22, 20, 26);
+            for (int e : {18, 20, 21, 22, 23, 24, 25, 26}) {
+                if (e > top) break;
                 int c = round_cap(1ull << e);
                 if (caps.empty() || caps.back() != c) caps.push_back(c);
             }
+        }
         std::vector<int> blocks;
         if (p->block_size > 0) blocks.push_back(p->block_size);
         else blocks = {64, 128, 256, 512};
-        alloc_paths(run.pb, (size_t)*std::max_element(caps.begin(), caps.end()), (size_t)run.total_pixels, true);
-        // several waves in flight: one pool per lane of the first scene
+        // several waves in flight: one pool per lane of the first scene, kept on its
+        // handle across calls (reallocated only when a batch needs more)
+        const size_t max_cap = (size_t)*std::max_element(caps.begin(), caps.end());
         const int npools = run.total_paths > (uint64_t)caps[0] ? std::min(scenes[0]->streams, kBatchPools) : 1;
         ensure_lane_streams(scenes[0], npools);
-        for (int k = 1; k < npools; k++)
-            run.add_pool((size_t)*std::max_element(caps.begin(), caps.end()), lane_stream(scenes[0], k));
+        for (int k = 0; k < npools; k++) ensure_batch_pool(scenes[0], k, max_cap, run.total_pixels);
+        run.pbs[0] = &scenes[0]->batch_pb[0];
+        for (int k = 1; k < npools; k++) run.add_pool(&scenes[0]->batch_pb[k], lane_stream(scenes[0], k));
         int best_cap = caps[0], best_block = blocks.size() == 1 ? blocks[0] : 256;
         double tune_ms = 0;
         if (caps.size() > 1 || blocks.size() > 1) {
@@ -1093,10 +1108,10 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
         HIP_OK(hipEventRecord(ev[0], run.st));
         run.run(ge, W, ev.data() + 1);
-        launch_batch_finalize(B, run.pb.P, run.pb.out, run.total_pixels, run.st);
+        launch_batch_finalize(B, run.pbs[0]->P, run.pbs[0]->out, run.total_pixels, run.st);
         HIP_OK(hipGetLastError());
         for (int i = 0; i < n; i++)
-            HIP_OK(hipMemcpyAsync(out_rgb[i], run.pb.out + 3 * run.pix_off[i], (run.pix_off[i + 1] - run.pix_off[i]) * 12,
+            HIP_OK(hipMemcpyAsync(out_rgb[i], run.pbs[0]->out + 3 * run.pix_off[i], (run.pix_off[i + 1] - run.pix_off[i]) * 12,
                                   hipMemcpyDeviceToHost, run.st));
         HIP_OK(hipStreamSynchronize(run.st));
         auto t1 = std::chrono::steady_clock::now();

@@ -487,8 +487,23 @@ struct SrcRays { // ray-query API: (org, tmin, dir, tmax) records -> prim id + t
 // Refill once 24 of the 64 lanes are idle; test parked leaves once 16 lanes hold
 // one (room 1024x768x256, chunked kernels: refill at 8 / 16 / 24 / 32 idle lanes
 // gives 401 / 433 / 444 / 440 Mpaths/s; leaf batches of 8 / 24 / 32 lose 1-2%).
-static constexpr int kRefillLanes = 24;
-static constexpr int kLeafBatch = 16;
+#ifndef TPT_REFILL_LANES
+#define TPT_REFILL_LANES 24
+#endif
+#ifndef TPT_LEAF_BATCH
+#define TPT_LEAF_BATCH 16
+#endif
+// leaf batches also start once fewer than this many lanes have inner nodes left
+// (0: only at kLeafBatch parked leaves or no inner work).  Room 1024x768x256,
+// depth 8: 0 / 16 / 24 give 612-619 / 618 / 622-624 Mpaths/s (extend alone 2.108
+// / 2.060 / 2.067 ms per launch); testing a lane's second leaf in the same batch
+// (instead of leaving the lane blocked until the next) lost 3%, refills at 16 or
+// 32 idle lanes 2-3%.
+#ifndef TPT_LEAF_INNER_MIN
+#define TPT_LEAF_INNER_MIN 24
+#endif
+static constexpr int kRefillLanes = TPT_REFILL_LANES;
+static constexpr int kLeafBatch = TPT_LEAF_BATCH;
 static constexpr int32_t kNoNode = -1; // never a leaf reference (those stay above 0xC0000000)
 
 // Inner-node visit only: `cur` becomes the nearest hit child (inner node or
@@ -600,7 +615,8 @@ __device__ __forceinline__ void persistent_trace(const DScene &S, const Src &src
                 }
                 uint64_t lm = __ballot(ray >= 0 && leaf != kNoNode);
                 uint64_t im = __ballot(ray >= 0 && cur >= 0);
-                if (lm != 0 && (__popcll(lm) >= kLeafBatch || im == 0)) {
+                if (lm != 0 && (__popcll(lm) >= kLeafBatch || im == 0 ||
+                                (TPT_LEAF_INNER_MIN > 0 && __popcll(im) < TPT_LEAF_INNER_MIN))) {
                     if (ray >= 0 && leaf != kNoNode) {
                         bool done = false;
                         leaf_tests(S, leaf, o, d, ac, best, ANY, tris, COUNT, done);
