@@ -243,11 +243,13 @@ struct DCache {
     int32_t train_cap;    // capacity of the training-record list
     float lo[3];          // scene bounds: position encoding (p - lo) * inv_ext
     float inv_ext[3];
-    int32_t suffix;       // > 0: training paths terminate into the cache `suffix` vertices after `depth`
-    int32_t pad_;
+    int32_t suffix;       // > 0: training paths terminate into the cache `suffix` vertices after their cache vertex
+    float min_rough;      // a path ends in the cache at its first vertex of depth >= `depth` whose first
+                          // lobe's roughness is >= min_rough (0: at depth `depth`, whatever the material)
     uint16_t *feat;       // per slot: 64 BF16 inputs of the vertex where the path queries the cache
     uint16_t *feat_t;     // per slot: 64 BF16 inputs of a training path's cache vertex
-    float4 *rec_T;        // per slot: throughput arriving at the vertex (training paths)
+    float4 *rec_T;        // per slot: throughput arriving at the vertex, w = its depth (-1: none yet)
+                          // (training paths)
     float4 *rec_C;        // per slot: radiance after the vertex's emission (training paths)
     int32_t *q_slot;      // query list (slots), count in counters[0]
     int32_t *t_slot;      // training list (slots), count in counters[1]

@@ -982,27 +982,36 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
         const DMaterial &m = S.materials[mi];
         color = add(color, cmul(decay, color_at(S, m.emission, sh.uv)));
         bool into_cache = false;
-        // the cache vertex, and the end of a training path's suffix (only training
-        // paths get that far: the others end at the cache vertex)
-        const bool at_cache = C && depth == C->depth;
-        const bool suffix_end = C && C->suffix > 0 && depth == C->depth + C->suffix;
-        if (at_cache || suffix_end) {
-            V3 hp = at(o, d, h.t);
-            V3 p01{(hp.x - C->lo[0]) * C->inv_ext[0], (hp.y - C->lo[1]) * C->inv_ext[1],
-                   (hp.z - C->lo[2]) * C->inv_ext[2]};
+        // The cache vertex: the first vertex of depth >= C->depth whose first lobe is
+        // rough enough (smooth mirrors and glass send the path on, Mueller et al.
+        // 2021).  A path ends there and queries the cache; a training path records
+        // the vertex and goes on, and queries the cache `suffix` vertices later.
+        if (C && depth >= C->depth) {
             float rough;
             V3 alb;
             lobe_summary(S, m, sh.uv, rough, alb);
-            const bool trains = at_cache && cache_trains(*C, slot);
-            nrc_encode((trains ? C->feat_t : C->feat) + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal, rough,
-                       alb);
-            if (trains) {
+            const bool eligible = rough >= C->min_rough;
+            const bool trains = cache_trains(*C, slot);
+            // depth of the training path's recorded vertex (none yet at the first candidate depth)
+            const int rec = trains && depth > C->depth ? (int)C->rec_T[slot].w : -1;
+            const bool record = trains && rec < 0 && eligible;
+            const bool query = (!trains && eligible) || (trains && rec >= 0 && C->suffix > 0 && depth == rec + C->suffix);
+            if (record || query) {
+                V3 hp = at(o, d, h.t);
+                V3 p01{(hp.x - C->lo[0]) * C->inv_ext[0], (hp.y - C->lo[1]) * C->inv_ext[1],
+                       (hp.z - C->lo[2]) * C->inv_ext[2]};
+                nrc_encode((record ? C->feat_t : C->feat) + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal,
+                           rough, alb);
+            }
+            if (record) {
                 out_.cache_kind = 2;
-                C->rec_T[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
+                C->rec_T[slot] = make_float4(decay.x, decay.y, decay.z, (float)depth);
                 C->rec_C[slot] = make_float4(color.x, color.y, color.z, 0.f);
-            } else {
+            } else if (query) {
                 out_.cache_kind = 1;
                 into_cache = true; // P.decay[slot] keeps the throughput arriving here
+            } else if (trains && depth == C->depth) {
+                C->rec_T[slot] = make_float4(0.f, 0.f, 0.f, -1.f); // nothing recorded yet
             }
         }
         if (depth < R.max_depth && !into_cache) {

@@ -202,16 +202,20 @@ __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t *__restrict__ 
 static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per wavefront
 
 // ---------------------------------------------------------------- training
-// Forward + backward over one 64-sample tile per wavefront (one wavefront per
-// workgroup: the tile's five activation matrices and two gradient matrices fill
-// ~64 KB of LDS).  Loss: relative L2, sum_c (p_c - t_c)^2 / (sg(p_c)^2 + 0.01),
+// Forward + backward over one 64-sample tile per workgroup of four wavefronts
+// (the tile's five activation matrices and two gradient matrices fill ~64 KB of
+// LDS).  The tile's 4 x 4 output blocks of 16 x 16 are split over the waves: in
+// the forward pass and the input gradient wave w computes feature block w of
+// every sample block, in the weight gradient input-feature block w of every
+// output block.  Every sum runs over K in the same groups of 8 as with one
+// wavefront.  Loss: relative L2, sum_c (p_c - t_c)^2 / (sg(p_c)^2 + 0.01),
 // averaged over the n samples.  Weight gradients are summed into grad (FP32,
 // W[l][in][out] layout) with atomics; the mean loss is added to *loss.
-__global__ __launch_bounds__(64) void k_nrc_train(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
-                                                  const float4 *__restrict__ target, const uint32_t *__restrict__ n_dev,
-                                                  int cap, int base, int n_max, const uint16_t *__restrict__ wt,
-                                                  const uint16_t *__restrict__ wb, float *__restrict__ grad,
-                                                  float *__restrict__ loss) {
+static constexpr int kTrainWaves = 4;
+__global__ __launch_bounds__(64 * kTrainWaves) void k_nrc_train(
+    const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx, const float4 *__restrict__ target,
+    const uint32_t *__restrict__ n_dev, int cap, int base, int n_max, const uint16_t *__restrict__ wt,
+    const uint16_t *__restrict__ wb, float *__restrict__ grad, float *__restrict__ loss) {
     __shared__ __attribute__((aligned(16))) uint16_t act[5][64 * kRow]; // layer inputs x0..x4
     __shared__ __attribute__((aligned(16))) uint16_t g[2][64 * kRow];   // output gradients (ping-pong)
     // records base .. base + n - 1 of the first min(*n_dev, cap): the producer keeps
@@ -219,98 +223,115 @@ __global__ __launch_bounds__(64) void k_nrc_train(const uint16_t *__restrict__ f
     const int n = min(max(min((int)*n_dev, cap) - base, 0), n_max);
     if (idx) idx += base;
     target += base;
-    const int l = threadIdx.x;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r = l & 15, q = l >> 4;
     const int tile0 = blockIdx.x * 64;
-    if (tile0 >= n) return;
-    load_tile(act[0], feats, idx, tile0, n);
+    if (tile0 >= n) return; // workgroup-uniform
+    // the tile's feature rows (zeros beyond n), 16 B per thread and step
+    for (int i = threadIdx.x; i < 64 * 8; i += 64 * kTrainWaves) {
+        const int s = i >> 3, c = (i & 7) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (tile0 + s < n) v = *reinterpret_cast<const uint4 *>(rec_row(feats, idx, tile0 + s) + c);
+        *reinterpret_cast<uint4 *>(act[0] + s * kRow + c) = v;
+    }
     __syncthreads();
-    float pred[4][4];
+    // hidden layers: wave w computes outputs w * 16 .. w * 16 + 15 of all 64 samples
     for (int ly = 0; ly < 4; ly++) {
-        layer_fwd<false>(act[ly], act[ly + 1], wt + ly * 4096, pred);
+        const uint16_t *x = act[ly];
+        const uint16_t *W = wt + ly * 4096;
+        f32x4 acc[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const bf16x8 b = ld8(W + (w * 16 + r) * 64 + ks * 32 + q * 8);
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) acc[mt] = mfma(ld8_lds(x + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
+        }
+        uint16_t *y = act[ly + 1];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) y[(mt * 16 + q * 4 + j) * kRow + w * 16 + r] = f2bf(fmaxf(acc[mt][j], 0.f));
         __syncthreads();
     }
-    layer_fwd<true>(act[4], nullptr, wt + 4 * 4096, pred);
+    // output layer (outputs 0..2 in block 0): wave w computes samples w * 16 .. w * 16 + 15
+    f32x4 pred = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+        pred = mfma(ld8_lds(act[4] + (w * 16 + r) * kRow + ks * 32 + q * 8), ld8(wt + 4 * 4096 + r * 64 + ks * 32 + q * 8),
+                    pred);
     // loss gradient of the output (columns 0..2), written as BF16 into g[0][s][0..31]
     // (columns 3..31 zero: the last layer's K step for the backward pass is 32)
-    for (int i = l; i < 64 * 32; i += 64) g[0][(i >> 5) * kRow + (i & 31)] = 0;
+    for (int i = threadIdx.x; i < 64 * 32; i += 64 * kTrainWaves) g[0][(i >> 5) * kRow + (i & 31)] = 0;
     __syncthreads();
     float lsum = 0.f;
     const float inv_n = 1.f / (float)n;
     if (r < 3) {
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int s = mt * 16 + q * 4 + j;
-                float gp = 0.f;
-                if (tile0 + s < n) {
-                    float4 t4 = target[tile0 + s];
-                    float t = r == 0 ? t4.x : (r == 1 ? t4.y : t4.z);
-                    float p = pred[mt][j];
-                    float den = p * p + 0.01f;
-                    float d = p - t;
-                    lsum += d * d / den * inv_n;
-                    gp = 2.f * d / den * inv_n;
-                }
-                g[0][s * kRow + r] = f2bf(gp);
+        for (int j = 0; j < 4; j++) {
+            const int s = w * 16 + q * 4 + j;
+            float gp = 0.f;
+            if (tile0 + s < n) {
+                const float4 t4 = target[tile0 + s];
+                const float t = r == 0 ? t4.x : (r == 1 ? t4.y : t4.z);
+                const float p = pred[j];
+                const float den = p * p + 0.01f;
+                const float d = p - t;
+                lsum += d * d / den * inv_n;
+                gp = 2.f * d / den * inv_n;
             }
+            g[0][s * kRow + r] = f2bf(gp);
+        }
     }
-    // wave-reduce the loss
     for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
     if (l == 0) atomicAdd(loss, lsum);
     __syncthreads();
-    // backward, last layer first: gout = g[cur] is dL/d(output of layer ly)
+    // backward, last layer first: go = g[cur] is dL/d(output of layer ly)
     int cur = 0;
     for (int ly = 4; ly >= 0; ly--) {
         const uint16_t *x = act[ly];
         const uint16_t *go = g[cur];
-        const int NO = ly == 4 ? 1 : 4; // output tiles of 16 (the last layer's 3 outputs sit in tile 0)
+        const int NO = ly == 4 ? 1 : 4; // output blocks of 16 (the last layer's 3 outputs sit in block 0)
         const int KO = ly == 4 ? 1 : 2; // K steps of 32 over the outputs
-        // dW[in][out] = sum_s x[s][in] * go[s][out]   (M = in, N = out, K = samples)
-        for (int mt = 0; mt < 4; mt++) {
+        // dW[in][out] = sum_s x[s][in] * go[s][out]  (M = in: block w, N = out, K = samples)
+        {
             f32x4 acc[4];
             for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                bf16x8 a = gather8(x + (ks * 32 + q * 8) * kRow + mt * 16 + r, kRow);
-                for (int nt = 0; nt < NO; nt++) {
-                    bf16x8 b = gather8(go + (ks * 32 + q * 8) * kRow + nt * 16 + r, kRow);
-                    acc[nt] = mfma(a, b, acc[nt]);
-                }
+                const bf16x8 a = gather8(x + (ks * 32 + q * 8) * kRow + w * 16 + r, kRow);
+                for (int nt = 0; nt < NO; nt++)
+                    acc[nt] = mfma(a, gather8(go + (ks * 32 + q * 8) * kRow + nt * 16 + r, kRow), acc[nt]);
             }
             for (int nt = 0; nt < NO; nt++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    int in = mt * 16 + q * 4 + j, out = nt * 16 + r;
+                    const int in = w * 16 + q * 4 + j, out = nt * 16 + r;
                     if (ly == 4 && out >= 3) continue;
                     atomicAdd(grad + ly * 4096 + in * 64 + out, acc[nt][j]);
                 }
         }
         if (ly == 0) break;
-        // gin[s][in] = (sum_o go[s][o] * W[in][o]) * (x[s][in] > 0)   (M = samples, N = in, K = out)
+        // gin[s][in] = (sum_o go[s][o] * W[in][o]) * (x[s][in] > 0)  (M = samples, N = in: block w, K = out)
         uint16_t *gi = g[cur ^ 1];
         const uint16_t *wl = wb + ly * 4096;
-        for (int mt = 0; mt < 4; mt++) {
-            f32x4 acc[4];
-            for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-            for (int ks = 0; ks < KO; ks++) {
-                bf16x8 a = ld8_lds(go + (mt * 16 + r) * kRow + ks * 32 + q * 8);
+        f32x4 acc[4];
 #pragma unroll
-                for (int nt = 0; nt < 4; nt++) {
-                    bf16x8 b = ld8(wl + (nt * 16 + r) * 64 + ks * 32 + q * 8);
-                    acc[nt] = mfma(a, b, acc[nt]);
-                }
-            }
+        for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < KO; ks++) {
+            const bf16x8 b = ld8(wl + (w * 16 + r) * 64 + ks * 32 + q * 8);
 #pragma unroll
-            for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int s = mt * 16 + q * 4 + j, in = nt * 16 + r;
-                    float relu_on = bf2f(x[s * kRow + in]) > 0.f ? 1.f : 0.f;
-                    gi[s * kRow + in] = f2bf(acc[nt][j] * relu_on);
-                }
+            for (int mt = 0; mt < 4; mt++) acc[mt] = mfma(ld8_lds(go + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
         }
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int s = mt * 16 + q * 4 + j, in = w * 16 + r;
+                const float relu_on = bf2f(x[s * kRow + in]) > 0.f ? 1.f : 0.f;
+                gi[s * kRow + in] = f2bf(acc[mt][j] * relu_on);
+            }
         __syncthreads();
         cur ^= 1;
     }
@@ -351,7 +372,8 @@ void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t 
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
                       int cap, int base, int n_max, const NrcWeights &W, hipStream_t st) {
     if (n_max <= 0) return;
-    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64), 0, st, feats, idx, target, n_dev, cap, base,
+    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64 * kTrainWaves), 0, st, feats, idx, target, n_dev,
+                       cap, base,
                        n_max, W.wt, W.wb, W.grad, W.loss);
 }
 void nrc_launch_adam(const NrcWeights &W, float lr, int step, hipStream_t st) {
