@@ -55,11 +55,11 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # the workload they were taken on.
 # Room 1024x768, 256 spp, depth 8: `bench.py --steps 1 --warmup 0 --no-cpu --no-count --no-cache`
 # (117 dispatches).
-EXTEND_COUNTERS = {"valu_insts": 1119544484, "thread_cycles": 37428053039, "active_inst": 1124401248,
-                   "fetch_kib": 1643711.6, "write_kib": 306683.7}
+EXTEND_COUNTERS = {"valu_insts": 1085295525, "thread_cycles": 37099849767, "active_inst": 1090375650,
+                   "fetch_kib": 1637813.2, "write_kib": 306847.6}
 # City 1920x1080, depth 8: one 16-spp render (18 dispatches, waves of 8 samples per pixel).
-CITY_EXTEND_COUNTERS = {"valu_insts": 1380013100, "thread_cycles": 49691613932, "active_inst": 1383996680,
-                        "fetch_kib": 3085776.4, "write_kib": 262941.7}
+CITY_EXTEND_COUNTERS = {"valu_insts": 1373123544, "thread_cycles": 49709677644, "active_inst": 1377392649,
+                        "fetch_kib": 3081072.1, "write_kib": 263549.7}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 DEFAULT_ROOM = (1024, 768, 256)
 
