@@ -143,7 +143,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--workload", choices=["room", "batch", "spheres", "cache", "city"], default="room")
     ap.add_argument("--ref-spp", type=int, default=16384, help="cache, city: samples per pixel of the reference")
-    ap.add_argument("--cache-suffix", type=int, default=3,
+    ap.add_argument("--cache-suffix", type=int, default=1,
                     help="cache: training paths end in the cache this many vertices past their cache vertex "
                          "(0: they run to the maximum depth)")
     ap.add_argument("--cache-frames", type=str, default="16,32,64,128,256", help="cache: frame counts of the rows")

@@ -251,7 +251,7 @@ def scene_info(scene_hip):
 # ---------------------------------------------------------------- learned radiance cache
 def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3, reset=False,
                   weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=4,
-                  suffix_len=3):
+                  suffix_len=1):
     """Progressive render with the learned radiance cache (cache_depth < 0: plain
     progressive frames).  suffix_len > 0 ends training paths in the cache too,
     suffix_len vertices past their cache vertex (self-training); 0 traces them
@@ -281,7 +281,7 @@ class ProgressiveRender:
 
     def __init__(self, scene_hip, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False,
                  cache_depth=-1, train_records=1 << 16, learning_rate=1e-3, reset=False, weight_seed=0,
-                 train_steps=4, suffix_len=3):
+                 train_steps=4, suffix_len=1):
         self._scene = scene_hip
         cam = scene_hip.camera()
         self._shape = (cam.height, cam.width, 3)
