@@ -500,7 +500,13 @@ struct SrcRays { // ray-query API: (org, tmin, dir, tmax) records -> prim id + t
 // / 2.060 / 2.067 ms per launch); testing a lane's second leaf in the same batch
 // (instead of leaving the lane blocked until the next) lost 3%, refills at 16 or
 // 32 idle lanes 2-3%, and leaf batches of 12 or 20 (with an inner minimum of 24,
-// 28 or 32) stayed within 1%.
+// 28 or 32) stayed within 1%.  Spreading a batch's triangle tests over the wave
+// (job slots by a prefix of the leaf sizes, owner lanes found by binary search
+// over shuffles, a segmented (t, gid) min reduction; same images) raised the
+// active lanes per VALU instruction from 33.7 to 40.4 but issued 5% more VALU
+// instructions: leaves hold 1.86 triangles on average, so the loop below runs
+// about two passes per batch and a batch is ~28% of the kernel's instructions,
+// while the inner-node iterations (45.6 lanes busy on average) are ~65%.
 #ifndef TPT_LEAF_INNER_MIN
 #define TPT_LEAF_INNER_MIN 24
 #endif
