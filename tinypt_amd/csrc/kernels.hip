@@ -1094,6 +1094,13 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
 
 // CACHE: the radiance-cache hooks are on; queries and training records are
 // appended to C's lists (at most C.train_cap training records).
+// Entries are shaded in queue order.  Sorting each workgroup's entries by
+// material class first (class bits in the hit record, an LDS counting sort; same
+// images) was slower on room 1024x768x256: the next queue inherits the sorted
+// order, so extend's lanes lose their ray coherence (2.05 -> 2.20 ms per
+// launch), and the sort's LDS and barriers cost shade more than the lane
+// divergence it removes (209 -> 219 ms per render alone; three streams 322 ->
+// 380 ms, the shade workgroups' LDS now competing with extend's stacks).
 template <bool CACHE>
 __global__ TPT_SHADE_ATTR void k_shade(DScene S, DRender R, DPaths P, const int32_t *queue, int32_t *next, int qsel, int nsel,
                         DCache C) {
