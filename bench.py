@@ -467,7 +467,7 @@ def cache_comparison(args, frame_counts, width=1920, height=1080):
     return {
         "workload": f"textured_room_264612tri_{W}x{H}_depth12_rgba8_alpha_bump",
         "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation ({ref_s:.1f} s)",
-        "cache": ("5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, 4 steps/frame, cache after 2 bounces, "
+        "cache": ("5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, 8 steps/frame, cache after 2 bounces, "
                   f"training suffixes end in the cache after {args.cache_suffix} more vertices"),
         "plain_relmse_times_s": round(c, 5),
         "plain": [dict(spp=s, ms=round(dt * 1e3, 1), relmse=round(e, 5)) for s, dt, e in plain],

@@ -2,6 +2,8 @@
 NumPy FP32 reference of the same network (weights and inter-layer activations
 rounded to BF16 as the kernels store them, FP32 accumulation), and the
 progressive renderer built on them."""
+import inspect
+
 import numpy as np
 import pytest
 
@@ -155,7 +157,7 @@ def test_cached_render_trains_online(hip, suffix_len):
     plain, _ = hip.render_cached(dev, 64, cache_depth=-1, seed=1, **kw)
     # 12,288 pixels, 2048 training records per frame (about one path in six), 4 Adam steps per frame
     img, st = hip.render_cached(dev, 64, cache_depth=2, train_records=2048, learning_rate=1e-3, reset=True,
-                                weight_seed=7, seed=1, suffix_len=suffix_len, **kw)
+                                weight_seed=7, seed=1, suffix_len=suffix_len, train_steps=4, **kw)
     print(st)
     assert np.all(np.isfinite(img)) and img.min() >= 0.0
     assert st["queries"] > 5000 and 0 < st["records"] <= 2048
@@ -169,7 +171,8 @@ def test_cached_render_trains_online(hip, suffix_len):
     assert abs(img.mean() - ref.mean()) < 0.05 * ref.mean()  # no visible bias
     assert rel(img) < rel(plain)  # lower error than the plain estimator at the same frames
     # the network keeps training across calls
-    _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, suffix_len=suffix_len, **kw)
+    _, st2 = hip.render_cached(dev, 2, cache_depth=2, train_records=2048, seed=2, suffix_len=suffix_len,
+                               train_steps=4, **kw)
     assert st2["steps"] == 64 * 4 + 2 * 4
 
 
@@ -200,7 +203,9 @@ def test_progressive_session_with_cache(hip):
     dev = scenes.triangle_room(128, 96).to("hip")
     kw = dict(max_depth=8, rr_depth=5, seed=1)
     ref = hip.PathTracer("hip").render(dev, 1024, hdr=True, fp64=True, max_depth=8, rr_depth=5, seed=99).raw()
-    sess = hip.ProgressiveRender(dev, cache_depth=2, train_records=2048, reset=True, weight_seed=7, **kw)
+    # 2048 records per frame in 4 Adam steps (mini-batches of 512; the default 8
+    # steps suit 2^16 records, mini-batches of 8192)
+    sess = hip.ProgressiveRender(dev, cache_depth=2, train_records=2048, reset=True, weight_seed=7, train_steps=4, **kw)
     for _ in range(4):
         sess.add(16)
     sess.wait()
@@ -239,7 +244,8 @@ def test_cached_frames_on_several_streams(hip, monkeypatch, lanes):
     one, many = out["1"], out[lanes]
     assert np.array_equal(one["plain"], many["plain"]) and np.array_equal(one["split"], many["plain"])
     assert np.array_equal(one["fixed"], many["fixed"])
-    assert one["st"]["steps"] == many["st"]["steps"] == 24 * 4
+    k = inspect.signature(hip.render_cached).parameters["train_steps"].default
+    assert one["st"]["steps"] == many["st"]["steps"] == 24 * k
     assert one["st"]["queries"] == many["st"]["queries"]
     # trained runs are stochastic in the record order (mean |diff| / mean between
     # runs measured 0.9e-2 .. 2.5e-2), so each is held to a bias bar against a
@@ -282,5 +288,7 @@ def test_weights_read_after_session_frames(hip):
     assert np.array_equal(w1, w2)
     # a cached render on the same scene trains after the session's steps
     _, st = hip.render_cached(dev, 4, cache_depth=2, train_records=1024, max_depth=6, seed=3)
-    assert st["steps"] == 16 * 4 + 4 * 4
+    k = inspect.signature(hip.render_cached).parameters["train_steps"].default
+    assert k == inspect.signature(hip.ProgressiveRender).parameters["train_steps"].default
+    assert st["steps"] == 16 * k + 4 * k
     sess.close()

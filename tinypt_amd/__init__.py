@@ -250,12 +250,14 @@ def scene_info(scene_hip):
 
 # ---------------------------------------------------------------- learned radiance cache
 def render_cached(scene_hip, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3, reset=False,
-                  weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=4,
+                  weight_seed=0, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False, train_steps=8,
                   suffix_len=1):
     """Progressive render with the learned radiance cache (cache_depth < 0: plain
     progressive frames).  suffix_len > 0 ends training paths in the cache too,
     suffix_len vertices past their cache vertex (self-training); 0 traces them
-    to max_depth.  Returns (H x W x 3 float32 HDR
+    to max_depth.  train_steps: Adam steps per frame, on consecutive
+    mini-batches of the frame's train_records records (textured room 1920x1080,
+    256 frames: 4 / 8 / 16 steps give equal-error speedups 2.30 / 2.42 / 2.25).  Returns (H x W x 3 float32 HDR
     mean, row 0 at the bottom, stats dict)."""
     import numpy as np
 
@@ -281,7 +283,7 @@ class ProgressiveRender:
 
     def __init__(self, scene_hip, max_depth=5, rr_depth=-1, shadow_bounded=False, seed=0, fp64=False,
                  cache_depth=-1, train_records=1 << 16, learning_rate=1e-3, reset=False, weight_seed=0,
-                 train_steps=4, suffix_len=1):
+                 train_steps=8, suffix_len=1):
         self._scene = scene_hip
         cam = scene_hip.camera()
         self._shape = (cam.height, cam.width, 3)
