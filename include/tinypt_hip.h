@@ -294,16 +294,19 @@ typedef struct {
     double render_ms;      /* wall time of the render */
     double infer_ms;       /* device time of the MLP inference launches */
     double train_ms;       /* device time of the MLP training + Adam launches */
-    double last_loss;      /* mean relative-L2 training loss of the last frame */
-    uint64_t queries;      /* paths terminated into the cache in the last frame */
-    uint64_t records;      /* training records of the last frame */
+    double last_loss;      /* mean relative-L2 training loss of the last frame group's steps */
+    uint64_t queries;      /* paths terminated into the cache per frame of the last frame group */
+    uint64_t records;      /* training records per frame of the last frame group */
     int32_t steps;         /* Adam steps taken so far by this scene's network */
     int32_t pad_;
 } tpt_cache_stats;
 /* Progressive render: `cache->frames` frames of one sample per pixel (frame f
  * uses sample index f of the per-(pixel, sample) streams of params->seed); after
  * each frame the cache queries are evaluated, the frame's training records
- * become targets and the network takes one Adam step.  Writes the HDR mean of
+ * become targets and the network takes train_steps Adam steps (environment
+ * TPT_FRAME_GROUP = 2..4 traces that many frames per pipeline wave, sharing the
+ * weights of the training before them; cache_depth < 0 groups frames up to about
+ * 2^22 paths by default, which leaves the image unchanged).  Writes the HDR mean of
  * the frames (H*W*3 floats, row 0 at the bottom); params->spp and output are
  * ignored, accum_fp64 selects FP64 accumulation.  `stats` may be NULL. */
 int tpt_hip_render_cached(tpt_hip_scene *scene, const tpt_render_params *params, const tpt_cache_params *cache,

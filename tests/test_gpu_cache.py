@@ -145,6 +145,27 @@ def test_progressive_without_cache_equals_plain_render(hip, dev):
     assert st["queries"] == 0 and st["records"] == 0
 
 
+@pytest.mark.parametrize("fp64", [False, True])
+def test_frame_groups_leave_plain_frames_unchanged(hip, dev, monkeypatch, fp64):
+    """Plain frames traced several per pipeline wave (TPT_FRAME_GROUP) are the
+    same samples, added to the sums in frame order: the image equals one frame
+    per wave bit for bit, also for a frame count the group does not divide and
+    for a session split over calls."""
+    kw = dict(cache_depth=-1, max_depth=6, rr_depth=3, seed=5, fp64=fp64)
+    out = {}
+    for g in ("1", "3"):
+        monkeypatch.setenv("TPT_FRAME_GROUP", g)
+        img, _ = hip.render_cached(dev, 7, **kw)
+        sess = hip.ProgressiveRender(dev, max_depth=6, rr_depth=3, seed=5, fp64=fp64)
+        sess.add(2).add(5)
+        split, n = sess.read()
+        sess.close()
+        assert n == 7
+        out[g] = (img, split)
+    assert np.array_equal(out["1"][0], out["3"][0])
+    assert np.array_equal(out["1"][0], out["1"][1]) and np.array_equal(out["1"][0], out["3"][1])
+
+
 @pytest.mark.parametrize("suffix_len", [0, 3])
 def test_cached_render_trains_online(hip, suffix_len):
     """suffix_len 0: training paths run to max_depth; 3: they end in the cache

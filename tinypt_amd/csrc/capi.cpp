@@ -42,7 +42,7 @@ void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hi
 void launch_cache_reset(const DCache &, hipStream_t);
 void launch_cache_apply(const DPaths &, const DCache &, const float4 *, int, hipStream_t);
 void launch_cache_targets(const DPaths &, const DCache &, float4 *, hipStream_t);
-void launch_frame_accumulate(const DPaths &, int, int, int, hipStream_t);
+void launch_frame_accumulate(const DPaths &, int, int, int, int, hipStream_t);
 void launch_frame_finalize(const DPaths &, int, int, int, float *, hipStream_t);
 void launch_fill_u32(uint32_t *, int, uint32_t, hipStream_t);
 } // namespace tpt
@@ -1255,12 +1255,15 @@ struct FrameSetup {
     DCache C{};
     tpt_cache_params cp{};
     bool use_cache = false;
-    int pixels = 0, grid = 0, fp64 = 0;
-    int lanes = 1; // frame f runs on lane f % lanes: its own path pool, stream and cache staging buffers
+    int pixels = 0, fp64 = 0;
+    int group = 1;    // frames traced together in one pipeline wave (group * pixels paths per launch)
+    size_t slots = 0; // group * pixels: path slots of a lane's pool
+    int lanes = 1;    // frame groups rotate over the lanes: path pools, streams and cache staging buffers
     uint64_t base_mixed = 0;
     int last_bounce = 0; // last bounce a path can reach (later bounces would trace empty queues)
 };
 constexpr int kFrameBlock = 256;
+constexpr int kMaxFrameGroup = 4;
 
 // Orders frames that run on different streams: `acc` is recorded after the
 // latest frame's accumulation (the per-pixel sums are shared and added in frame
@@ -1269,6 +1272,8 @@ constexpr int kFrameBlock = 256;
 struct FrameChain {
     hipEvent_t acc = nullptr, start = nullptr;
     bool any = false; // a frame has been enqueued: later frames wait on the events
+    int next_lane = 0, last_lane = 0; // lane of the next frame group, of the latest one
+    int last_frames = 0;              // frames of the latest group
     FrameChain() = default;
     FrameChain(const FrameChain &) = delete;
     FrameChain &operator=(const FrameChain &) = delete;
@@ -1306,7 +1311,21 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
     fs.R.spp_eff = 1;
     fs.R.output = 1;
     fs.R.k = 1;
-    fs.grid = grid_for((size_t)fs.pixels, kFrameBlock, h->cus);
+    // Frames per pipeline wave: one frame of a 1920x1080 image is 2M paths, and
+    // launches that small lose to their fill and drain, so plain frames are
+    // traced in groups of up to about 4M paths (at most kMaxFrameGroup;
+    // TPT_FRAME_GROUP overrides).  A group's frames are the same samples as one
+    // by one and accumulate in frame order: without the cache the image does not
+    // depend on the grouping.  Textured room 1920x1080, 64 plain frames on three
+    // streams: groups of 1 / 2 / 4 take 341 / 323 / 317 ms.  With the cache a
+    // group's frames share the weights of the training before the group, and
+    // groups do not pay (256 cached frames: 675 / 672 / 693 ms, the frames bound
+    // by their three bounces' path kernels), so cached frames go one by one.
+    const int gauto = fs.use_cache ? 1
+                                   : (int)std::min<size_t>(kMaxFrameGroup,
+                                                           std::max<size_t>(1, ((size_t)1 << 22) / fs.pixels));
+    fs.group = env_int("TPT_FRAME_GROUP", gauto, 1, kMaxFrameGroup);
+    fs.slots = (size_t)fs.group * fs.pixels;
     fs.base_mixed = splitmix64(p->seed);
     fs.fp64 = p->accum_fp64 ? 1 : 0;
     fs.lanes = std::min(h->streams, kMaxLanes);
@@ -1314,7 +1333,7 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
     if (fs.use_cache) {
         cache_alloc_weights(h);
         if (cp->reset || h->cache.step == 0) cache_init_weights(h, cp->weight_seed);
-        for (int l = 0; l < fs.lanes; l++) cache_alloc_buffers(h, (size_t)fs.pixels, cp->train_records, l);
+        for (int l = 0; l < fs.lanes; l++) cache_alloc_buffers(h, fs.slots, fs.group * cp->train_records, l);
         DCache &C = fs.C;
         C.depth = cp->cache_depth;
         // enough training candidates that about train_records of them reach the cache vertex
@@ -1354,16 +1373,16 @@ DCache lane_cache(const DCache &C0, const CacheBufs &b) {
     return C;
 }
 
-// Enqueues frames [f0, f0 + n), adding each frame to the per-pixel sums that
-// the pools share (frame 0 starts them).  Frame f runs on the pool and stream
-// of lane f % lanes; a frame's inference waits for the previous frame's
-// training and its accumulation for the previous accumulation, so the result
-// equals one stream's.
-// `ev`: 4 events per frame bracketing the inference and the training launches,
-// or null.  `clear_loss`: the last frame's training loss is accumulated from
-// zero (read back by the caller).
-void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[kMaxLanes], FrameChain &ch, int f0,
-                    int n, hipEvent_t *ev, bool clear_loss) {
+// Enqueues frames [f0, f0 + n) in groups of up to fs.group frames, adding each
+// frame to the per-pixel sums that the pools share (frame 0 starts them).
+// Groups rotate over the lanes' pools and streams; a group's inference waits
+// for the previous group's training and its accumulation for the previous
+// accumulation, so the result equals one stream's.  Returns the number of groups.
+// `ev`: 4 events per group bracketing the inference and the training launches,
+// or null (room for n groups).  `clear_loss`: the last group's training loss is
+// accumulated from zero (read back by the caller).
+int enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[kMaxLanes], FrameChain &ch, int f0,
+                   int n, hipEvent_t *ev, bool clear_loss) {
     const DScene &S = h->S;
     CacheState &c = h->cache;
     ch.create();
@@ -1372,44 +1391,52 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[k
         HIP_OK(hipEventRecord(ch.start, h->stream));
         for (int l = 1; l < fs.lanes; l++) HIP_OK(hipStreamWaitEvent(lane_stream(h, l), ch.start, 0));
     }
-    for (int f = f0; f < f0 + n; f++) {
-        const int lane = f % fs.lanes;
+    int groups = 0;
+    for (int f = f0; f < f0 + n; groups++) {
+        const int g = std::min(fs.group, f0 + n - f); // frames f .. f + g - 1
+        const int lane = ch.next_lane;
+        ch.next_lane = (lane + 1) % fs.lanes;
+        ch.last_lane = lane;
+        ch.last_frames = g;
         hipStream_t st = lane_stream(h, lane);
         DPaths &P = pools[lane]->P;
         CacheBufs &cb = c.buf[lane];
-        hipEvent_t *e = ev ? &ev[4 * (size_t)(f - f0)] : nullptr;
+        hipEvent_t *e = ev ? &ev[4 * (size_t)groups] : nullptr;
+        const int np = g * fs.pixels;
+        const int grid = grid_for((size_t)np, kFrameBlock, h->cus);
         DCache C{};
         if (fs.use_cache) {
             C = lane_cache(fs.C, cb);
             C.frame = (uint32_t)(c.step + f);
+            C.train_cap = g * fs.cp.train_records;
             launch_cache_reset(C, st);
         }
-        launch_generate(S, fs.R, P, f, fs.pixels, fs.base_mixed, kFrameBlock, st);
+        launch_generate(S, fs.R, P, f, np, fs.base_mixed, kFrameBlock, st); // slot j: frame f + j / pixels
         int cur = 0;
         for (int b = 0; b <= fs.last_bounce; b++) {
             int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
             int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
-            launch_extend(S, P, q, cur, 1 - cur, fs.grid, kFrameBlock, false, (uint64_t)fs.pixels, st);
-            launch_shade(S, fs.R, P, q, nx, cur, 1 - cur, fs.grid, kFrameBlock, st, fs.use_cache ? &C : nullptr);
-            launch_shadow(S, P, fs.grid, kFrameBlock, false, (uint64_t)fs.pixels, st);
+            launch_extend(S, P, q, cur, 1 - cur, grid, kFrameBlock, false, (uint64_t)np, st);
+            launch_shade(S, fs.R, P, q, nx, cur, 1 - cur, grid, kFrameBlock, st, fs.use_cache ? &C : nullptr);
+            launch_shadow(S, P, grid, kFrameBlock, false, (uint64_t)np, st);
             cur = 1 - cur;
         }
         if (fs.use_cache) {
             // inference reads the weights of the scene's latest training step, and
-            // this frame's training must follow it (any session or cached render)
+            // this group's training must follow it (any session or cached render)
             if (h->train_any) HIP_OK(hipStreamWaitEvent(st, h->train_ev, 0));
-            if (clear_loss && f + 1 == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
+            if (clear_loss && f + g == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
             if (e) HIP_OK(hipEventRecord(e[0], st));
-            nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, fs.pixels, c.W.wt, cb.pred, st);
+            nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, np, c.W.wt, cb.pred, st);
             if (e) HIP_OK(hipEventRecord(e[1], st));
-            launch_cache_apply(P, C, cb.pred, fs.pixels, st);
+            launch_cache_apply(P, C, cb.pred, np, st);
             launch_cache_targets(P, C, cb.target, st);
             if (e) HIP_OK(hipEventRecord(e[2], st));
-            // train_steps Adam steps on consecutive mini-batches of the frame's records
+            // train_steps Adam steps per frame, on consecutive mini-batches of
+            // train_records / train_steps of the group's records
             const int K = fs.cp.train_steps, mb = (fs.cp.train_records + K - 1) / K;
-            for (int k = 0; k < K; k++) {
-                nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, fs.cp.train_records, k * mb, mb,
-                                 c.W, st);
+            for (int k = 0; k < g * K; k++) {
+                nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, C.train_cap, k * mb, mb, c.W, st);
                 nrc_launch_adam(c.W, fs.cp.learning_rate, ++c.step, st);
             }
             if (e) HIP_OK(hipEventRecord(e[3], st));
@@ -1417,10 +1444,12 @@ void enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[k
             h->train_any = true;
         }
         if (ch.any) HIP_OK(hipStreamWaitEvent(st, ch.acc, 0));
-        launch_frame_accumulate(P, fs.pixels, fs.fp64, f == 0, st);
+        launch_frame_accumulate(P, fs.pixels, fs.fp64, f == 0, g, st);
         HIP_OK(hipEventRecord(ch.acc, st));
         ch.any = true;
+        f += g;
     }
+    return groups;
 }
 
 } // namespace
@@ -1434,24 +1463,24 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         FrameSetup fs;
         setup_frames(h, p, cp, fs);
         const int pixels = fs.pixels;
-        ensure_paths(h, (size_t)pixels, pixels);
-        if (fs.lanes > 1) ensure_paths2(h, (size_t)pixels, pixels);
+        ensure_paths(h, fs.slots, pixels);
+        if (fs.lanes > 1) ensure_paths2(h, fs.slots, pixels);
         for (int l = 2; l < fs.lanes; l++) {
             PathBuffers &x = h->pbx[l - 2];
-            if (!(x.mem && x.slots >= (size_t)pixels && x.pixels == pixels)) {
+            if (!(x.mem && x.slots >= fs.slots && x.pixels == pixels)) {
                 sync_lanes(h);
-                alloc_paths(x, (size_t)pixels, (size_t)pixels, false);
+                alloc_paths(x, fs.slots, (size_t)pixels, false);
             }
             share_sums(x, h->pb);
         }
         PathBuffers *pools[kMaxLanes] = {&h->pb, &h->pb2, &h->pbx[0], &h->pbx[1]};
         hipStream_t st = h->stream;
         CacheState &c = h->cache;
-        // four events per frame (inference and training brackets), read after the last frame
+        // four events per frame group (inference and training brackets), read after the last
         EventVec ev(fs.use_cache ? 4 * (size_t)cp->frames : 0);
         for (auto &x : ev) HIP_OK(hipEventCreate(&x));
         FrameChain ch;
-        enqueue_frames(h, fs, pools, ch, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
+        const int groups = enqueue_frames(h, fs, pools, ch, 0, cp->frames, ev.empty() ? nullptr : ev.data(), true);
         HIP_OK(hipStreamWaitEvent(st, ch.acc, 0)); // the last frame may have run on another lane's stream
         launch_frame_finalize(h->pb.P, pixels, fs.fp64, cp->frames, h->pb.out, st);
         HIP_OK(hipGetLastError());
@@ -1459,13 +1488,13 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
         uint32_t cnt[2] = {0, 0};
         float loss = 0.f;
         if (fs.use_cache) {
-            const CacheBufs &last = c.buf[(cp->frames - 1) % fs.lanes];
+            const CacheBufs &last = c.buf[ch.last_lane];
             HIP_OK(hipMemcpyAsync(cnt, last.counters, 8, hipMemcpyDeviceToHost, st));
             HIP_OK(hipMemcpyAsync(&loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
         }
         sync_lanes(h);
         double infer_ms = 0, train_ms = 0;
-        for (size_t f = 0; 4 * f < ev.size(); f++) {
+        for (size_t f = 0; !ev.empty() && f < (size_t)groups; f++) {
             float a = 0, b2 = 0;
             HIP_OK(hipEventElapsedTime(&a, ev[4 * f], ev[4 * f + 1]));
             HIP_OK(hipEventElapsedTime(&b2, ev[4 * f + 2], ev[4 * f + 3]));
@@ -1477,9 +1506,11 @@ int tpt_hip_render_cached(tpt_hip_scene *h, const tpt_render_params *p, const tp
             stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             stats->infer_ms = infer_ms;
             stats->train_ms = train_ms;
-            stats->last_loss = fs.use_cache ? loss / cp->train_steps : 0.0; // sum of the mini-batch means
-            stats->queries = fs.use_cache ? cnt[0] : 0;
-            stats->records = fs.use_cache ? std::min<uint32_t>(cnt[1], (uint32_t)cp->train_records) : 0;
+            // the last group's sums, per frame
+            const int g = std::max(1, ch.last_frames);
+            stats->last_loss = fs.use_cache ? loss / (g * cp->train_steps) : 0.0; // mean of the mini-batch means
+            stats->queries = fs.use_cache ? cnt[0] / g : 0;
+            stats->records = fs.use_cache ? std::min<uint32_t>(cnt[1], (uint32_t)(g * cp->train_records)) / g : 0;
             stats->steps = c.step;
         }
         return TPT_OK;
@@ -1522,9 +1553,9 @@ int tpt_hip_progress_create(tpt_hip_scene *scene, const tpt_render_params *param
         s = new tpt_hip_progress();
         setup_frames(scene, params, cache, s->fs);
         s->scene = scene;
-        alloc_paths(s->pb, (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+        alloc_paths(s->pb, s->fs.slots, (size_t)s->fs.pixels, false);
         for (int l = 1; l < s->fs.lanes; l++) {
-            alloc_paths(s->lanes[l - 1], (size_t)s->fs.pixels, (size_t)s->fs.pixels, false);
+            alloc_paths(s->lanes[l - 1], s->fs.slots, (size_t)s->fs.pixels, false);
             share_sums(s->lanes[l - 1], s->pb);
         }
         *out = s;

@@ -1295,15 +1295,31 @@ __global__ void k_cache_targets(DPaths P, DCache C, float4 *target) {
     target[i] = make_float4(cache_target(f.x, a.x, T.x), cache_target(f.y, a.y, T.y), cache_target(f.z, a.z, T.z), 0.f);
 }
 
-// One-sample-per-pixel frame (slot = pixel) added to the running per-pixel sums.
-__global__ void k_frame_accumulate(DPaths P, int pixels, int fp64, int first) {
+// `frames` one-sample-per-pixel frames (frame g's pixel p in slot g * pixels + p)
+// added to the running per-pixel sums one after the other, in frame order.
+__global__ void k_frame_accumulate(DPaths P, int pixels, int fp64, int first, int frames) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
-    float4 L = P.color[p];
-    const float v[3] = {L.x, L.y, L.z};
-    for (int c = 0; c < 3; c++) {
-        if (fp64) P.pix_d[3 * p + c] = (first ? 0.0 : P.pix_d[3 * p + c]) + (double)v[c];
-        else P.pix_f[3 * p + c] = (first ? 0.f : P.pix_f[3 * p + c]) + v[c];
+    if (fp64) {
+        double s[3];
+        for (int c = 0; c < 3; c++) s[c] = first ? 0.0 : P.pix_d[3 * p + c];
+        for (int g = 0; g < frames; g++) {
+            const float4 L = P.color[(size_t)g * pixels + p];
+            s[0] += (double)L.x;
+            s[1] += (double)L.y;
+            s[2] += (double)L.z;
+        }
+        for (int c = 0; c < 3; c++) P.pix_d[3 * p + c] = s[c];
+    } else {
+        float s[3];
+        for (int c = 0; c < 3; c++) s[c] = first ? 0.f : P.pix_f[3 * p + c];
+        for (int g = 0; g < frames; g++) {
+            const float4 L = P.color[(size_t)g * pixels + p];
+            s[0] += L.x;
+            s[1] += L.y;
+            s[2] += L.z;
+        }
+        for (int c = 0; c < 3; c++) P.pix_f[3 * p + c] = s[c];
     }
 }
 __global__ void k_frame_finalize(DPaths P, int pixels, int fp64, int frames, float *out) {
@@ -2020,8 +2036,8 @@ void launch_cache_apply(const DPaths &P, const DCache &C, const float4 *pred, in
 void launch_cache_targets(const DPaths &P, const DCache &C, float4 *target, hipStream_t st) {
     hipLaunchKernelGGL(k_cache_targets, dim3((C.train_cap + 255) / 256), dim3(256), 0, st, P, C, target);
 }
-void launch_frame_accumulate(const DPaths &P, int pixels, int fp64, int first, hipStream_t st) {
-    hipLaunchKernelGGL(k_frame_accumulate, dim3((pixels + 255) / 256), dim3(256), 0, st, P, pixels, fp64, first);
+void launch_frame_accumulate(const DPaths &P, int pixels, int fp64, int first, int frames, hipStream_t st) {
+    hipLaunchKernelGGL(k_frame_accumulate, dim3((pixels + 255) / 256), dim3(256), 0, st, P, pixels, fp64, first, frames);
 }
 void launch_frame_finalize(const DPaths &P, int pixels, int fp64, int frames, float *out, hipStream_t st) {
     hipLaunchKernelGGL(k_frame_finalize, dim3((pixels + 255) / 256), dim3(256), 0, st, P, pixels, fp64, frames, out);
