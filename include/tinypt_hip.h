@@ -249,7 +249,7 @@ typedef struct {
     uint64_t paths;         /* total paths of the batch */
     int32_t queue_capacity; /* path slots per pipeline wave (tuned or given) */
     int32_t block_size;     /* threads per workgroup of generate / shade (tuned or given) */
-    int32_t waves;          /* pipeline waves = replays of the captured hipGraph */
+    int32_t waves;          /* pipeline waves: replays of the captured hipGraph + plain waves of large scenes */
     int32_t pad_;
 } tpt_batch_stats;
 /* Renders n scenes (all created on the current device with the same node layout, n
@@ -258,6 +258,9 @@ typedef struct {
  * replayed until all paths are done, alternating between up to three pools on as
  * many streams (one with TPT_STREAMS=1); each wave's accumulation follows the
  * previous wave's, so the images are bit-identical to single-scene renders.
+ * Scenes of at least 2^23 paths (TPT_BATCH_LARGE=10..40 sets the exponent) fill
+ * the GPU alone: they run after the batched ones as plain pipeline waves of their
+ * own (tpt_hip_render's kernels and 2^24-slot waves) on the same pools.
  * `params` gives the shared settings (max_depth, rr_depth, shadow_bounded, output,
  * accum_fp64); its spp and seed are ignored (per scene in `items`).
  * params->queue_capacity == 0 selects the capacity by autotuning over 2^18..2^22

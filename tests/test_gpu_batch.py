@@ -101,3 +101,23 @@ def test_batch_pools_reused_across_calls(hip, batch):
                                queue_capacity=cap, block_size=128, **kw)
         for i, img in zip(idx, imgs):
             assert np.array_equal(img.raw(), ref[i]), f"scene {i} differs (batch {idx}, capacity {cap})"
+
+
+@pytest.mark.parametrize("large_log2,streams", [(13, "3"), (13, "1"), (10, "3"), (13, "2")])
+def test_batch_large_scenes_as_plain_waves(hip, batch, monkeypatch, large_log2, streams):
+    # scenes of at least 2^TPT_BATCH_LARGE paths run as plain pipeline waves of their
+    # own after the batched ones (2^13: the spheres box and the feature zoo; 2^10:
+    # every scene, no batched waves); the images still equal the single renders
+    hosts, _, spps, seeds = batch
+    monkeypatch.setenv("TPT_STREAMS", streams)
+    devs = [h.to("hip") for h in hosts]
+    monkeypatch.setenv("TPT_BATCH_LARGE", str(large_log2))
+    kw = dict(max_depth=6, rr_depth=3, hdr=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    pt = hip.PathTracer("hip")
+    for cap, block in ((4096, 64), (0, 0)):
+        imgs = pt.render_batch(devs, spps, seeds, queue_capacity=cap, block_size=block, **kw)
+        st = pt.last_batch_stats()
+        assert len(st["scene_ms"]) == len(devs) and all(t > 0 for t in st["scene_ms"])
+        for i, (a, b) in enumerate(zip(imgs, ref)):
+            assert np.array_equal(a.raw(), b), f"scene {i} differs (2^{large_log2}, capacity {cap})"

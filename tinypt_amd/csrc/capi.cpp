@@ -235,6 +235,7 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     S.env_gcond = (const int32_t *)(base + o_gc);
     S.light_shapes = (const int2 *)(base + o_ls);
     S.layout = H.layout;
+    S.loc_bits = H.loc_bits;
     const char *pers = std::getenv("TPT_PERSISTENT");
     // traversal kernels (room 1024x768x256, Mpaths/s):
     //   0 one ray per lane; a wave keeps its 64 consecutive queue entries until all are done: 331
@@ -490,6 +491,15 @@ DRender make_render(const tpt_render_params *p, int spp, uint64_t seed, int pixe
     return R;
 }
 
+// Samples per pixel of one pipeline wave: as many as `cap` path slots hold.
+// (Spreading the samples evenly over the waves instead, 1024x768 at 256 spp in
+// 2^24 slots as 13 waves of 20 / 16 rather than 12 of 21 and one of 4, measured
+// 605-615 against 614-625 Mpaths/s on three streams: the short last wave
+// overlaps the others' tails.)
+int wave_samples(int spp_eff, size_t cap, size_t pixels) {
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)spp_eff, cap / pixels));
+}
+
 int grid_for(size_t n, int block, int cus = 256) {
     size_t g = (n + block - 1) / block;
     // one thread per item up to 65536 threads per CU (grid-stride beyond): the
@@ -502,6 +512,66 @@ int grid_for(size_t n, int block, int cus = 256) {
     if (g < 1) g = 1;
     return (int)g;
 }
+
+// Pipeline waves of one plain render (generate, max_depth + 1 x extend / shade /
+// shadow, accumulate) over `npools` path pools on as many streams: wave w runs on
+// pool (lane0 + w) % npools.  The waves share the per-pixel sums, so each wave's
+// accumulate waits for the previous one's (`prev_acc`: index into `evs` of that
+// accumulate's event, -1 before the first wave; updated) and the samples are added
+// in order.  With `timing`, events bracket every extend / shade / shadow launch.
+struct PlainWaves {
+    DPaths pools[kMaxLanes] = {};
+    hipStream_t streams[kMaxLanes] = {};
+    int npools = 1, block = 256, cus = 256;
+    bool count = false, timing = false;
+    std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
+    int waves = 0;
+    static int record(EventVec &evs, hipStream_t s, bool timing) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreateWithFlags(&e, timing ? hipEventDefault : hipEventDisableTiming));
+        evs.push_back(e);
+        HIP_OK(hipEventRecord(e, s));
+        return (int)evs.size() - 1;
+    }
+    // returns the pool of the last wave
+    int enqueue(const DScene &S, const DRender &R, int k, uint64_t base_mixed, int lane0, EventVec &evs, int &prev_acc) {
+        const int pixels = R.pixels;
+        int lane = lane0;
+        for (int wave_base = 0, w = 0; wave_base < R.spp_eff; wave_base += k, w++) {
+            lane = (lane0 + w) % npools;
+            hipStream_t sw = streams[lane];
+            const DPaths &P = pools[lane];
+            auto ev = [&]() { return record(evs, sw, timing); };
+            int k_active = std::min(k, R.spp_eff - wave_base);
+            int n = pixels * k_active;
+            launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);
+            int grid = grid_for((size_t)n, block, cus);
+            int cur = 0;
+            for (int b = 0; b <= R.max_depth; b++) {
+                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
+                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+                int e0 = timing ? ev() : -1;
+                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, sw);
+                int e1 = timing ? ev() : -1;
+                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, sw, nullptr);
+                int e2 = timing ? ev() : -1;
+                launch_shadow(S, P, grid, block, count, (uint64_t)n, sw);
+                int e3 = timing ? ev() : -1;
+                if (timing) {
+                    ext_ev.push_back({e0, e1});
+                    shade_ev.push_back({e1, e2});
+                    shadow_ev.push_back({e2, e3});
+                }
+                cur = 1 - cur;
+            }
+            if (npools > 1 && prev_acc >= 0) HIP_OK(hipStreamWaitEvent(sw, evs[prev_acc], 0));
+            launch_accumulate(R, P, wave_base, k_active, w == 0 ? 1 : 0, sw);
+            if (npools > 1) prev_acc = record(evs, sw, false);
+            waves++;
+        }
+        return lane;
+    }
+};
 
 } // namespace
 
@@ -587,7 +657,7 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
         int pixels = W * Hh;
         DRender R = make_render(p, p->spp, p->seed, pixels);
         size_t cap = p->queue_capacity > 0 ? (size_t)p->queue_capacity : ((size_t)1 << 24);
-        int k = (int)std::max<size_t>(1, std::min<size_t>((size_t)R.spp_eff, cap / (size_t)pixels));
+        const int k = wave_samples(R.spp_eff, cap, (size_t)pixels);
         R.k = k;
         int block = p->block_size > 0 ? p->block_size : 256;
         if (block % 64 != 0 || block > 1024) return set_err(TPT_ERR_INVALID_ARGUMENT, "block_size must be a multiple of 64 <= 1024");
@@ -620,40 +690,19 @@ int tpt_hip_render(tpt_hip_scene *h, const tpt_render_params *p, float *out_rgb,
             for (int l = 1; l < npools; l++) // the other pools start after this point of st
                 HIP_OK(hipStreamWaitEvent(lane_stream(h, l), evs[e], 0));
         }
-        std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
-        int first = 1, prev_acc = -1;
-        for (int wave_base = 0, w = 0; wave_base < R.spp_eff; wave_base += k, w++) {
-            const int lane = w % npools;
-            hipStream_t sw = lane_stream(h, lane);
-            DPaths &P = lane_pool(h, lane)->P;
-            auto ev = [&]() { return ev_on(sw); };
-            int k_active = std::min(k, R.spp_eff - wave_base);
-            int n = pixels * k_active;
-            launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);
-            int grid = grid_for((size_t)n, block, h->cus);
-            int cur = 0;
-            for (int b = 0; b <= R.max_depth; b++) {
-                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
-                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
-                int e0 = timing ? ev() : -1;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, sw);
-                int e1 = timing ? ev() : -1;
-                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, sw, nullptr);
-                int e2 = timing ? ev() : -1;
-                launch_shadow(S, P, grid, block, count, (uint64_t)n, sw);
-                int e3 = timing ? ev() : -1;
-                if (timing) {
-                    ext_ev.push_back({e0, e1});
-                    shade_ev.push_back({e1, e2});
-                    shadow_ev.push_back({e2, e3});
-                }
-                cur = 1 - cur;
-            }
-            if (two && prev_acc >= 0) HIP_OK(hipStreamWaitEvent(sw, evs[prev_acc], 0));
-            launch_accumulate(R, P, wave_base, k_active, first, sw);
-            if (two) prev_acc = ev_on(sw);
-            first = 0;
+        PlainWaves pw;
+        for (int l = 0; l < npools; l++) {
+            pw.pools[l] = lane_pool(h, l)->P;
+            pw.streams[l] = lane_stream(h, l);
         }
+        pw.npools = npools;
+        pw.block = block;
+        pw.cus = h->cus;
+        pw.count = count;
+        pw.timing = timing;
+        int prev_acc = -1;
+        pw.enqueue(S, R, k, base_mixed, 0, evs, prev_acc);
+        auto &ext_ev = pw.ext_ev, &shade_ev = pw.shade_ev, &shadow_ev = pw.shadow_ev;
         if (two) HIP_OK(hipStreamWaitEvent(st, evs[prev_acc], 0)); // the last accumulate may be on another stream
         launch_finalize(R, h->pb.P, h->pb.out, st);
         HIP_OK(hipGetLastError());
@@ -1013,8 +1062,24 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             run.path_off[i + 1] = run.path_off[i] + (uint64_t)px * (uint64_t)rend[i].spp_eff;
             run.pix_off[i + 1] = run.pix_off[i] + (uint64_t)px;
         }
-        run.total_paths = run.path_off[n];
-        run.total_pixels = run.pix_off[n];
+        // Scenes of at least 2^TPT_BATCH_LARGE paths (default 2^23; they sort last)
+        // fill the GPU on their own: they run as plain pipeline waves of their own
+        // (the single-scene kernels, no segment tables) on the batch's pools after
+        // the batched scenes; the smaller scenes [0, nb) share batched waves.
+        const uint64_t large = (uint64_t)1 << env_int("TPT_BATCH_LARGE", 23, 10, 40);
+        int nb = n;
+        while (nb > 0 && run.path_off[nb] - run.path_off[nb - 1] >= large) nb--;
+        const uint64_t all_pixels = run.pix_off[n];
+        run.total_paths = run.path_off[nb];
+        run.total_pixels = run.pix_off[nb];
+        // plain waves: as tpt_hip_render's default, up to 2^24 path slots
+        constexpr size_t kPlainCap = (size_t)1 << 24;
+        size_t plain_slots = 0;
+        for (int i = nb; i < n; i++) {
+            const size_t px = (size_t)rend[i].pixels;
+            rend[i].k = wave_samples(rend[i].spp_eff, kPlainCap, px);
+            plain_slots = std::max(plain_slots, px * (size_t)rend[i].k);
+        }
         // tables in one allocation
         size_t off = 0;
         auto take = [&](size_t bytes) {
@@ -1041,13 +1106,15 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         B.seed_mixed = (const uint64_t *)(m + o_sd);
         B.seg = (int32_t *)(m + o_sg);
         B.state = (uint64_t *)(m + o_st);
-        B.n = n;
+        B.n = nb;
         B.seg_stride = seg_stride_for(n);
         B.pools = 1;
         // candidate capacities: 2^18, 2^20, 2^21, 2^22 path slots (the batch
         // configuration's range; TPT_BATCH_MAX_CAP raises the top to 2^23 .. 2^26),
         // never (much) more than the batch needs
-        auto round_cap = [&](uint64_t c) { return (int)std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256); };
+        auto round_cap = [&](uint64_t c) {
+            return (int)std::max<uint64_t>(256, std::min<uint64_t>(c, (run.total_paths + 255) / 256 * 256));
+        };
         std::vector<int> caps;
         if (p->queue_capacity > 0) caps.push_back(round_cap((uint64_t)p->queue_capacity));
         else {
@@ -1063,15 +1130,15 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         else blocks = {64, 128, 256, 512};
         // several waves in flight: one pool per lane of the first scene, kept on its
         // handle across calls (reallocated only when a batch needs more)
-        const size_t max_cap = (size_t)*std::max_element(caps.begin(), caps.end());
-        const int npools = run.total_paths > (uint64_t)caps[0] ? std::min(scenes[0]->streams, kBatchPools) : 1;
+        const size_t max_cap = std::max((size_t)*std::max_element(caps.begin(), caps.end()), plain_slots);
+        const int npools = run.total_paths > (uint64_t)caps[0] || nb < n ? std::min(scenes[0]->streams, kBatchPools) : 1;
         ensure_lane_streams(scenes[0], npools);
-        for (int k = 0; k < npools; k++) ensure_batch_pool(scenes[0], k, max_cap, run.total_pixels);
+        for (int k = 0; k < npools; k++) ensure_batch_pool(scenes[0], k, max_cap, all_pixels);
         run.pbs[0] = &scenes[0]->batch_pb[0];
         for (int k = 1; k < npools; k++) run.add_pool(&scenes[0]->batch_pb[k], lane_stream(scenes[0], k));
         int best_cap = caps[0], best_block = blocks.size() == 1 ? blocks[0] : 256;
         double tune_ms = 0;
-        if (caps.size() > 1 || blocks.size() > 1) {
+        if (nb > 0 && (caps.size() > 1 || blocks.size() > 1)) {
             auto tt = std::chrono::steady_clock::now();
             double best = -1;
             // every trial times the same 2^27 paths around the batch's middle (the
@@ -1098,17 +1165,59 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             }
             tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
         }
-        // the render: one captured pipeline wave per pool, replayed until every path is done
+        // the render: one captured pipeline wave per pool, replayed until every path
+        // of the batched scenes is done; then the large scenes' plain waves
         auto tr = std::chrono::steady_clock::now();
         hipGraphExec_t ge[BatchRun::kMaxPools] = {};
-        run.capture(best_cap, best_block, ge);
-        run.reset(best_cap);
-        int W = run.waves(best_cap);
+        const int W = nb > 0 ? run.waves(best_cap) : 0;
         EventVec ev(W + 1);
         for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-        HIP_OK(hipEventRecord(ev[0], run.st));
-        run.run(ge, W, ev.data() + 1);
-        launch_batch_finalize(B, run.pbs[0]->P, run.pbs[0]->out, run.total_pixels, run.st);
+        if (nb > 0) {
+            run.capture(best_cap, best_block, ge);
+            run.reset(best_cap);
+            HIP_OK(hipEventRecord(ev[0], run.st));
+            run.run(ge, W, ev.data() + 1);
+            launch_batch_finalize(B, run.pbs[0]->P, run.pbs[0]->out, run.total_pixels, run.st);
+        } else {
+            HIP_OK(hipEventRecord(ev[0], run.st));
+        }
+        // large scenes: plain pipeline waves on the same pools, continuing the
+        // rotation; each scene's sums and image are its range of the pools' arrays
+        EventVec pev;
+        std::vector<int> done_ev(n, -1);
+        int plain_waves = 0;
+        if (nb < n) {
+            PlainWaves pw;
+            pw.npools = run.pools;
+            pw.block = 256;
+            pw.cus = run.num_cu;
+            for (int k = 0; k < run.pools; k++) pw.streams[k] = run.stream(k);
+            if (run.pools > 1) { // every pool's plain waves follow what is queued on st so far
+                const int e = PlainWaves::record(pev, run.st, false);
+                for (int k = 1; k < run.pools; k++) HIP_OK(hipStreamWaitEvent(run.stream(k), pev[e], 0));
+            }
+            int lane0 = W % run.pools;
+            for (int i = nb; i < n; i++) {
+                const size_t o3 = 3 * run.pix_off[i];
+                for (int k = 0; k < run.pools; k++) {
+                    DPaths P = run.pool(k).P;
+                    P.sub_f += o3, P.pix_f += o3, P.sub_d += o3, P.pix_d += o3;
+                    pw.pools[k] = P;
+                }
+                int prev = -1;
+                const int last = pw.enqueue(scenes[i]->S, rend[i], rend[i].k, seeds[i], lane0, pev, prev);
+                // the last accumulate runs on the last wave's stream after all earlier ones
+                launch_finalize(rend[i], pw.pools[last], run.pbs[0]->out + o3, run.stream(last));
+                done_ev[i] = PlainWaves::record(pev, run.stream(last), true);
+                lane0 = (last + 1) % run.pools;
+            }
+            plain_waves = pw.waves;
+            for (int k = 0; k < run.pools; k++) { // st continues after every pool's last plain wave
+                if (run.stream(k) == run.st) continue;
+                const int e = PlainWaves::record(pev, run.stream(k), false);
+                HIP_OK(hipStreamWaitEvent(run.st, pev[e], 0));
+            }
+        }
         HIP_OK(hipGetLastError());
         for (int i = 0; i < n; i++)
             HIP_OK(hipMemcpyAsync(out_rgb[i], run.pbs[0]->out + 3 * run.pix_off[i], (run.pix_off[i + 1] - run.pix_off[i]) * 12,
@@ -1117,21 +1226,26 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
         auto t1 = std::chrono::steady_clock::now();
         if (scene_ms) {
             for (int i = 0; i < n; i++) {
-                int w = (int)((run.path_off[i + 1] - 1) / (uint64_t)best_cap);
                 float ms = 0;
-                HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[w + 1]));
+                if (i < nb) {
+                    int w = (int)((run.path_off[i + 1] - 1) / (uint64_t)best_cap);
+                    HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[w + 1]));
+                } else {
+                    HIP_OK(hipEventElapsedTime(&ms, ev[0], pev[done_ev[i]]));
+                }
                 scene_ms[order[i]] = ms;
             }
         }
-        for (int k = 0; k < run.pools; k++) (void)hipGraphExecDestroy(ge[k]);
+        for (int k = 0; k < run.pools; k++) // none captured when every scene ran as plain waves
+            if (ge[k]) (void)hipGraphExecDestroy(ge[k]);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
             stats->render_ms = std::chrono::duration<double, std::milli>(t1 - tr).count();
             stats->tune_ms = tune_ms;
-            stats->paths = run.total_paths;
+            stats->paths = run.path_off[n];
             stats->queue_capacity = best_cap;
             stats->block_size = best_block;
-            stats->waves = W;
+            stats->waves = W + plain_waves;
         }
         (void)t0;
         return TPT_OK;

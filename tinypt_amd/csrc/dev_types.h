@@ -169,7 +169,9 @@ struct DScene {
     int32_t chunk;      // queue entries per wave of the chunked traversal kernels (persistent 3, 4)
     int32_t env_gh;     // guide buckets of the marginal CDF (a power of two)
     int32_t env_gw;     // guide buckets of each conditional CDF (a power of two)
-    int32_t pad5;
+    int32_t loc_bits;   // > 0: a triangle's location (leaf records, hit records) is
+                        // tri_id | material << loc_bits, so shading knows the material
+                        // with the hit; 0: the location is the tri_id alone
     DTex bg;
     float bg_rot[9]; // columns u, v, w
     DCamera cam;

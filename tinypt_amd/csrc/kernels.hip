@@ -70,6 +70,11 @@ struct HitRec {
 };
 
 // alpha test of a triangle candidate (object.cpp:273-287)
+// Triangle id of a triangle location (DScene::loc_bits: the material index may sit above it).
+__device__ __forceinline__ int32_t loc_tri(const DScene &S, int32_t loc) {
+    return S.loc_bits ? loc & (int32_t)((1u << S.loc_bits) - 1u) : loc;
+}
+
 __device__ __forceinline__ bool tri_alpha_pass(const DScene &S, int tri, float u, float v, const AlphaCtx &ac,
                                                int32_t gid) {
     int4 i0 = S.tri_info[2 * tri], i1 = S.tri_info[2 * tri + 1];
@@ -146,7 +151,7 @@ __device__ __forceinline__ void leaf_tests(const DScene &S, int32_t cur, V3 o, V
         int32_t gid = __float_as_int(r0.w);
         if (kEPS <= t && (t < best.t || (t == best.t && gid < best.gid))) {
             if (!any && ac.on && (__float_as_int(r2.w) & 1)) {
-                if (!tri_alpha_pass(S, __float_as_int(r1.w), u, v, ac, gid)) continue;
+                if (!tri_alpha_pass(S, loc_tri(S, __float_as_int(r1.w)), u, v, ac, gid)) continue;
             }
             best = HitRec{t, gid, u, v, __float_as_int(r1.w)};
             if (any) {
@@ -898,10 +903,12 @@ __device__ __forceinline__ void surface(const DScene &S, V3 o, V3 d, const HitRe
         }
         return;
     }
-    // triangle (object.cpp:266-331), from its shading record (one cache line)
-    const float4 *sr = S.shade_rec + (size_t)kShadeRec * loc;
+    // triangle (object.cpp:266-331), from its shading record (one cache line); the
+    // material index comes with the location when it fits there, so its record is
+    // read beside the shading record instead of after it
+    const float4 *sr = S.shade_rec + (size_t)kShadeRec * loc_tri(S, loc);
     const float4 r0 = sr[0], r1 = sr[1], r2 = sr[2], r3 = sr[3], r4 = sr[4], r5 = sr[5];
-    mat = __float_as_int(r0.w);
+    mat = S.loc_bits ? (int)((uint32_t)loc >> S.loc_bits) : __float_as_int(r0.w);
     const int flags = __float_as_int(r1.w);
     float u = h.u, v = h.v;
     V3 ab = mk4(r0), ac = mk4(r1);

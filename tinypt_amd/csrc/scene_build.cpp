@@ -446,6 +446,13 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
         H.tris_bvh.assign((size_t)(ntri + 1) * 3, make_float4(0.f, 0.f, 0.f, 0.f));
         H.tris_bvh[3 * (size_t)ntri] = make_float4(0.f, 0.f, 0.f, __int_as_float_host(INT32_MAX));
         const int32_t sentinel_leaf = (int32_t)(kLeafBit | ((uint32_t)ntri << 3));
+        // tri ids need loc_bits bits; the material index goes above them when it fits
+        // in the remaining 31 (the sign marks analytic locations)
+        {
+            int bits = 1;
+            while (bits < 31 && ((int64_t)1 << bits) < (int64_t)ntri) bits++;
+            H.loc_bits = bits < 31 && (int64_t)H.materials.size() <= ((int64_t)1 << (31 - bits)) ? bits : 0;
+        }
         int cursor = 0;
         std::vector<int> leaf_first(B.nodes.size(), -1);
         std::vector<int> stk2{0};
@@ -461,7 +468,8 @@ void build_host_scene(const tpt_scene_desc &d, HostScene &H) {
                     int mat = H.tri_info[2 * tri].x;
                     int flags = H.materials[mat].alpha_opaque ? 0 : 1;
                     H.tris_bvh[3 * cursor] = r0;
-                    H.tris_bvh[3 * cursor + 1] = make_float4(r1.x, r1.y, r1.z, __int_as_float_host(tri));
+                    const int32_t loc = H.loc_bits ? (int32_t)((uint32_t)tri | (uint32_t)mat << H.loc_bits) : tri;
+                    H.tris_bvh[3 * cursor + 1] = make_float4(r1.x, r1.y, r1.z, __int_as_float_host(loc));
                     H.tris_bvh[3 * cursor + 2] = make_float4(r2.x, r2.y, r2.z, __int_as_float_host(flags));
                     cursor++;
                 }

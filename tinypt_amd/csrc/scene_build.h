@@ -43,6 +43,7 @@ struct HostScene {
     int max_stack = 0;      // worst-case traversal stack entries of the flattened tree
     int layout = kNodeBvh4Q; // node layout (dev_types.h)
     int bfs_prefix = 0;      // 4-wide layouts: nodes [0, bfs_prefix) are the top of the tree in BFS order
+    int loc_bits = 0;        // leaf records carry tri_id | material << loc_bits (DScene::loc_bits)
 };
 
 // Number of 4-wide nodes laid out breadth-first at the start of the node array.
