@@ -1541,7 +1541,7 @@ int enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[kM
             if (h->train_any) HIP_OK(hipStreamWaitEvent(st, h->train_ev, 0));
             if (clear_loss && f + g == f0 + n) HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
             if (e) HIP_OK(hipEventRecord(e[0], st));
-            nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, np, c.W.wt, cb.pred, st);
+            nrc_launch_infer(cb.feat, cb.q_slot, cb.counters + 0, np, c.W.wt, cb.pred, h->cus, st);
             if (e) HIP_OK(hipEventRecord(e[1], st));
             launch_cache_apply(P, C, cb.pred, np, st);
             launch_cache_targets(P, C, cb.target, st);
@@ -1776,7 +1776,7 @@ int tpt_hip_cache_eval(tpt_hip_scene *h, const uint16_t *inputs, int32_t n, floa
         cache_stage(h, inputs, nullptr, n);
         CacheState &c = h->cache;
         const CacheBufs &b = c.buf[0];
-        nrc_launch_infer(b.feat, nullptr, b.counters, n, c.W.wt, b.pred, h->stream);
+        nrc_launch_infer(b.feat, nullptr, b.counters, n, c.W.wt, b.pred, h->cus, h->stream);
         HIP_OK(hipGetLastError());
         std::vector<float4> o(n);
         HIP_OK(hipMemcpyAsync(o.data(), b.pred, (size_t)n * 16, hipMemcpyDeviceToHost, h->stream));

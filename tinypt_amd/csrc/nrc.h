@@ -80,7 +80,7 @@ __device__ inline void nrc_encode(uint16_t *dst, V3 pos01, V3 wo, V3 n, float ro
 // feats: records of 64 BF16; `idx` (may be null) maps record i to row idx[i] of
 // feats; the record count is read from *n_dev (at most n_max).
 void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t *n_dev, int n_max,
-                      const uint16_t *wt, float4 *out, hipStream_t st);
+                      const uint16_t *wt, float4 *out, int num_cu, hipStream_t st);
 // Trains on records [base, base + n_max) of the first min(*n_dev, cap) records
 // (the only ones stored: the producer may count more).
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,

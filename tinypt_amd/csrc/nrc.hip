@@ -20,6 +20,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "nrc.h"
 
 namespace tpt {
@@ -122,84 +124,138 @@ __device__ __forceinline__ bf16x8 ld4x2(const uint16_t *p0, const uint16_t *p1) 
     __builtin_memcpy(&r, &v, 16);
     return r;
 }
+
+// Relu and BF16 rounding (to nearest even, v_cvt_pk_bf16_f32) of 2 x 4 FP32
+// values, packed as one B operand.  The ReLU runs on the packed BF16 pairs as a
+// signed 16-bit max with 0 (v_pk_max_i16): a BF16 value with the sign bit set
+// reads as a negative integer, so every negative value and -0 become +0, the
+// same as rounding max(v, 0).
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) short i16x2;
+__device__ __forceinline__ uint32_t relu_bf16x2(float a, float b) {
+    const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);
+    const i16x2 m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0});
+    return __builtin_bit_cast(uint32_t, m);
+}
 __device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
-    uint16_t v[8];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        v[j] = f2bf(fmaxf(lo[j], 0.f));
-        v[4 + j] = f2bf(fmaxf(hi[j], 0.f));
-    }
-    bf16x8 r;
-    __builtin_memcpy(&r, v, 16);
-    return r;
+    const uint4 v = make_uint4(relu_bf16x2(lo[0], lo[1]), relu_bf16x2(lo[2], lo[3]), relu_bf16x2(hi[0], hi[1]),
+                               relu_bf16x2(hi[2], hi[3]));
+    return __builtin_bit_cast(bf16x8, v);
 }
 
-// NT sample tiles of 16 per wavefront (tile of NT * 16 records).
+// Persistent inference: every wavefront loads the network's A operands once (the
+// four hidden layers' 4 x 2 tiles and the output layer's 2, 136 registers) and
+// then walks 16 * NT-record tiles of the records with a grid stride, so the
+// weights are read once per wavefront instead of once per tile.
 template <int NT>
 __global__ __launch_bounds__(256) void k_nrc_infer(const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx,
                                                    const uint32_t *__restrict__ n_dev, int n_max,
                                                    const uint16_t *__restrict__ wt, float4 *__restrict__ out) {
     const int n = rec_count(n_dev, n_max);
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-    const int tile0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (NT * 16);
-    if (tile0 >= n) return; // whole wavefront beyond the records (no barriers below)
-    // layer 0 operand: features s * 32 + g * 8 + {0..7} of sample nt * 16 + r
-    bf16x8 xb[NT][2];
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    if (wave * (NT * 16) >= n) return; // wavefront-uniform (no barriers below)
+    // A operands: wt rows (outputs) mt * 16 + r; layer 0 reads features s * 32 + g * 8
+    // + {0..7}, the later layers the permuted order of their register-held inputs
+    bf16x8 A[4][4][2], A4[2];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) {
-        const int i = tile0 + nt * 16 + r;
-        const uint16_t *row = rec_row(feats, idx, i < n ? i : tile0);
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-            xb[nt][s] = ld8(row + s * 32 + g * 8);
-            if (i >= n) xb[nt][s] = bf16x8{};
-        }
-    }
-#pragma unroll
-    for (int ly = 0; ly < 4; ly++) {
-        const uint16_t *W = wt + ly * 4096;
-        f32x4 acc[4][NT];
+    for (int ly = 0; ly < 4; ly++)
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-            for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) {
-                const uint16_t *row = W + (mt * 16 + r) * 64 + s * 32;
-                const bf16x8 a = ly == 0 ? ld8(row + g * 8) : ld4x2(row + g * 4, row + 16 + g * 4);
-#pragma unroll
-                for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma(a, xb[nt][s], acc[mt][nt]);
+            for (int s = 0; s < 2; s++) {
+                const uint16_t *row = wt + ly * 4096 + (mt * 16 + r) * 64 + s * 32;
+                A[ly][mt][s] = ly == 0 ? ld8(row + g * 8) : ld4x2(row + g * 4, row + 16 + g * 4);
             }
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const uint16_t *row = wt + 4 * 4096 + r * 64 + s * 32;
+        A4[s] = ld4x2(row + g * 4, row + 16 + g * 4);
+    }
+    // software pipeline over the wavefront's tiles: the record rows of the next
+    // tile and the row indices of the one after are loaded while a tile computes
+    const int stride = waves * (NT * 16);
+    auto row_of = [&](int tile0, int nt) { // row index of sample nt * 16 + r of a tile (-1 beyond n)
+        const int i = tile0 + nt * 16 + r;
+        return i < n ? (idx ? idx[i] : i) : -1;
+    };
+    auto load_x = [&](const int rows[NT], bf16x8 x[NT][2]) { // layer 0 operand of a tile
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const uint16_t *row = feats + (size_t)(rows[nt] < 0 ? 0 : rows[nt]) * 64;
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                x[nt][s] = ld8(row + s * 32 + g * 8);
+                if (rows[nt] < 0) x[nt][s] = bf16x8{};
+            }
+        }
+    };
+    int rows_next[NT];
+    bf16x8 x_next[NT][2];
+    {
+        int rows0[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) rows0[nt] = row_of(wave * (NT * 16), nt);
+        load_x(rows0, x_next);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) rows_next[nt] = row_of(wave * (NT * 16) + stride, nt);
+    }
+    for (int tile0 = wave * (NT * 16); tile0 < n; tile0 += stride) {
+        // layer 0 operand: features s * 32 + g * 8 + {0..7} of sample nt * 16 + r
+        bf16x8 xb[NT][2];
 #pragma unroll
         for (int nt = 0; nt < NT; nt++)
 #pragma unroll
-            for (int s = 0; s < 2; s++) xb[nt][s] = relu_pack(acc[2 * s][nt], acc[2 * s + 1][nt]);
-    }
-    // output layer: one 16-output tile (outputs 0..2 are the rgb prediction)
-    const uint16_t *W = wt + 4 * 4096;
-    f32x4 o[NT];
+            for (int s = 0; s < 2; s++) xb[nt][s] = x_next[nt][s];
+        if (tile0 + stride < n) {
+            load_x(rows_next, x_next);
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-        const uint16_t *row = W + r * 64 + s * 32;
-        const bf16x8 a = ld4x2(row + g * 4, row + 16 + g * 4);
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) o[nt] = mfma(a, xb[nt][s], o[nt]);
-    }
-    if (g == 0) // lanes 0..15 hold outputs 0..3 of sample nt * 16 + r
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-            const int i = tile0 + nt * 16 + r;
-            if (i < n) out[i] = make_float4(o[nt][0], o[nt][1], o[nt][2], 0.f);
+            for (int nt = 0; nt < NT; nt++) rows_next[nt] = row_of(tile0 + 2 * stride, nt);
         }
+#pragma unroll
+        for (int ly = 0; ly < 4; ly++) {
+            f32x4 acc[4][NT];
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma(A[ly][mt][s], xb[nt][s], acc[mt][nt]);
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+                for (int s = 0; s < 2; s++) xb[nt][s] = relu_pack(acc[2 * s][nt], acc[2 * s + 1][nt]);
+        }
+        // output layer: one 16-output tile (outputs 0..2 are the rgb prediction)
+        f32x4 o[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) o[nt] = mfma(A4[s], xb[nt][s], o[nt]);
+        if (g == 0) // lanes 0..15 hold outputs 0..3 of sample nt * 16 + r
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const int i = tile0 + nt * 16 + r;
+                if (i < n) out[i] = make_float4(o[nt][0], o[nt][1], o[nt][2], 0.f);
+            }
+    }
 }
 #ifndef TPT_NRC_INFER_TILES
 #define TPT_NRC_INFER_TILES 2
 #endif
-static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per wavefront
+static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per wavefront and step
+// resident inference wavefronts per CU (two per SIMD: the weights take 136 registers)
+#ifndef TPT_NRC_INFER_WAVES_PER_CU
+#define TPT_NRC_INFER_WAVES_PER_CU 8
+#endif
 
 // ---------------------------------------------------------------- training
 // Forward + backward over one 64-sample tile per workgroup of four wavefronts
@@ -362,11 +418,11 @@ __global__ void k_nrc_adam(float *__restrict__ w, float *__restrict__ m, float *
 
 // ---------------------------------------------------------------- launchers
 void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t *n_dev, int n_max,
-                      const uint16_t *wt, float4 *out, hipStream_t st) {
+                      const uint16_t *wt, float4 *out, int num_cu, hipStream_t st) {
     if (n_max <= 0) return;
     const int per_wave = kInferTiles * 16;
-    int waves = (n_max + per_wave - 1) / per_wave;
-    int blocks = (waves + 3) / 4;
+    const int waves = std::min((n_max + per_wave - 1) / per_wave, num_cu * TPT_NRC_INFER_WAVES_PER_CU);
+    const int blocks = (waves + 3) / 4;
     hipLaunchKernelGGL(k_nrc_infer<kInferTiles>, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
 }
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
