@@ -1269,7 +1269,8 @@ void cache_alloc_weights(tpt_hip_scene *h) {
         return o;
     };
     size_t o_w = take(4 * kNrcParams), o_m = take(4 * kNrcParams), o_v = take(4 * kNrcParams),
-           o_g = take(4 * kNrcParams), o_wt = take(2 * kNrcParams), o_wb = take(2 * kNrcParams), o_l = take(16);
+           o_g = take(4 * kNrcParams), o_wt = take(2 * kNrcParams), o_wb = take(2 * kNrcParams), o_l = take(16),
+           o_p = take(4 * (size_t)kNrcParams * kNrcMaxTrainGroups);
     HIP_OK(hipMalloc(&c.wmem, off));
     HIP_OK(hipMemset(c.wmem, 0, off));
     uint8_t *b = (uint8_t *)c.wmem;
@@ -1280,6 +1281,7 @@ void cache_alloc_weights(tpt_hip_scene *h) {
     c.W.wt = (uint16_t *)(b + o_wt);
     c.W.wb = (uint16_t *)(b + o_wb);
     c.W.loss = (float *)(b + o_l);
+    c.W.partial = (float *)(b + o_p);
 }
 
 // Uploads FP32 weights [l][in][out] with their BF16 copies and clears the Adam state.
@@ -1551,7 +1553,7 @@ int enqueue_frames(tpt_hip_scene *h, FrameSetup &fs, PathBuffers *const pools[kM
             const int K = fs.cp.train_steps, mb = (fs.cp.train_records + K - 1) / K;
             for (int k = 0; k < g * K; k++) {
                 nrc_launch_train(cb.feat_t, cb.t_slot, cb.target, cb.counters + 1, C.train_cap, k * mb, mb, c.W, st);
-                nrc_launch_adam(c.W, fs.cp.learning_rate, ++c.step, st);
+                nrc_launch_adam(c.W, mb, fs.cp.learning_rate, ++c.step, st);
             }
             if (e) HIP_OK(hipEventRecord(e[3], st));
             HIP_OK(hipEventRecord(h->train_ev, st));
@@ -1803,10 +1805,10 @@ int tpt_hip_cache_train(tpt_hip_scene *h, const uint16_t *inputs, const float *t
         hipStream_t st = h->stream;
         HIP_OK(hipMemsetAsync(c.W.loss, 0, 4, st));
         nrc_launch_train(b.feat, nullptr, b.target, b.counters + 1, n, 0, n, c.W, st);
+        nrc_launch_adam(c.W, n, lr, ++c.step, st); // sums the step's gradient into grad
         HIP_OK(hipGetLastError());
         if (grad) HIP_OK(hipMemcpyAsync(grad, c.W.grad, 4 * kNrcParams, hipMemcpyDeviceToHost, st));
         if (loss) HIP_OK(hipMemcpyAsync(loss, c.W.loss, 4, hipMemcpyDeviceToHost, st));
-        nrc_launch_adam(c.W, lr, ++c.step, st);
         HIP_OK(hipStreamSynchronize(st));
         return TPT_OK;
     } catch (const tpt_error &e) {

@@ -130,12 +130,9 @@ __device__ __forceinline__ bf16x8 ld4x2(const uint16_t *p0, const uint16_t *p1) 
 // signed 16-bit max with 0 (v_pk_max_i16): a BF16 value with the sign bit set
 // reads as a negative integer, so every negative value and -0 become +0, the
 // same as rounding max(v, 0).
-typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
-typedef __attribute__((ext_vector_type(2))) float f32x2;
 typedef __attribute__((ext_vector_type(2))) short i16x2;
 __device__ __forceinline__ uint32_t relu_bf16x2(float a, float b) {
-    const bf16x2 p = __builtin_convertvector(f32x2{a, b}, bf16x2);
-    const i16x2 m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0});
+    const i16x2 m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, bf16_pair(a, b)), i16x2{0, 0});
     return __builtin_bit_cast(uint32_t, m);
 }
 __device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
@@ -258,20 +255,27 @@ static constexpr int kInferTiles = TPT_NRC_INFER_TILES; // 16-record tiles per w
 #endif
 
 // ---------------------------------------------------------------- training
-// Forward + backward over one 64-sample tile per workgroup of four wavefronts
-// (the tile's five activation matrices and two gradient matrices fill ~64 KB of
-// LDS).  The tile's 4 x 4 output blocks of 16 x 16 are split over the waves: in
+// Forward + backward over 64-sample tiles on workgroups of four wavefronts (a
+// tile's five activation matrices and two gradient matrices fill ~64 KB of
+// LDS).  A tile's 4 x 4 output blocks of 16 x 16 are split over the waves: in
 // the forward pass and the input gradient wave w computes feature block w of
 // every sample block, in the weight gradient input-feature block w of every
 // output block.  Every sum runs over K in the same groups of 8 as with one
 // wavefront.  Loss: relative L2, sum_c (p_c - t_c)^2 / (sg(p_c)^2 + 0.01),
-// averaged over the n samples.  Weight gradients are summed into grad (FP32,
-// W[l][in][out] layout) with atomics; the mean loss is added to *loss.
+// averaged over the n samples; the mean loss is added to *loss.
+// A workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, .. and sums its
+// tiles' weight gradients into
+// partial[blockIdx.x] (W[l][in][out] layout; the output layer's padding columns
+// are not written, zero without a tile), and the Adam kernel sums the gridDim.x
+// partials in a fixed order.  (One FP32 atomic
+// per weight and tile into one gradient, 2.1M atomics per step of 8192 records,
+// took 38 us per step alone; the partials move 10 MB instead.)
+__device__ __forceinline__ uint16_t bf16_one(float f) { return (uint16_t)bf16_pair(f, 0.f); }
 static constexpr int kTrainWaves = 4;
 __global__ __launch_bounds__(64 * kTrainWaves) void k_nrc_train(
     const uint16_t *__restrict__ feats, const int32_t *__restrict__ idx, const float4 *__restrict__ target,
     const uint32_t *__restrict__ n_dev, int cap, int base, int n_max, const uint16_t *__restrict__ wt,
-    const uint16_t *__restrict__ wb, float *__restrict__ grad, float *__restrict__ loss) {
+    const uint16_t *__restrict__ wb, float *__restrict__ partial, float *__restrict__ loss) {
     __shared__ __attribute__((aligned(16))) uint16_t act[5][64 * kRow]; // layer inputs x0..x4
     __shared__ __attribute__((aligned(16))) uint16_t g[2][64 * kRow];   // output gradients (ping-pong)
     // records base .. base + n - 1 of the first min(*n_dev, cap): the producer keeps
@@ -281,128 +285,161 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_nrc_train(
     target += base;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r = l & 15, q = l >> 4;
-    const int tile0 = blockIdx.x * 64;
-    if (tile0 >= n) return; // workgroup-uniform
-    // the tile's feature rows (zeros beyond n), 16 B per thread and step
-    for (int i = threadIdx.x; i < 64 * 8; i += 64 * kTrainWaves) {
-        const int s = i >> 3, c = (i & 7) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (tile0 + s < n) v = *reinterpret_cast<const uint4 *>(rec_row(feats, idx, tile0 + s) + c);
-        *reinterpret_cast<uint4 *>(act[0] + s * kRow + c) = v;
-    }
-    __syncthreads();
-    // hidden layers: wave w computes outputs w * 16 .. w * 16 + 15 of all 64 samples
-    for (int ly = 0; ly < 4; ly++) {
-        const uint16_t *x = act[ly];
-        const uint16_t *W = wt + ly * 4096;
-        f32x4 acc[4];
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            const bf16x8 b = ld8(W + (w * 16 + r) * 64 + ks * 32 + q * 8);
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) acc[mt] = mfma(ld8_lds(x + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
-        }
-        uint16_t *y = act[ly + 1];
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) y[(mt * 16 + q * 4 + j) * kRow + w * 16 + r] = f2bf(fmaxf(acc[mt][j], 0.f));
-        __syncthreads();
-    }
-    // output layer (outputs 0..2 in block 0): wave w computes samples w * 16 .. w * 16 + 15
-    f32x4 pred = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 2; ks++)
-        pred = mfma(ld8_lds(act[4] + (w * 16 + r) * kRow + ks * 32 + q * 8), ld8(wt + 4 * 4096 + r * 64 + ks * 32 + q * 8),
-                    pred);
-    // loss gradient of the output (columns 0..2), written as BF16 into g[0][s][0..31]
-    // (columns 3..31 zero: the last layer's K step for the backward pass is 32)
-    for (int i = threadIdx.x; i < 64 * 32; i += 64 * kTrainWaves) g[0][(i >> 5) * kRow + (i & 31)] = 0;
-    __syncthreads();
+    // this workgroup's gradient: written by its first tile, added to by the others
+    float *dst = partial + (size_t)blockIdx.x * kNrcParams;
+    const float inv_n = n > 0 ? 1.f / (float)n : 0.f;
     float lsum = 0.f;
-    const float inv_n = 1.f / (float)n;
-    if (r < 3) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int s = w * 16 + q * 4 + j;
-            float gp = 0.f;
-            if (tile0 + s < n) {
-                const float4 t4 = target[tile0 + s];
-                const float t = r == 0 ? t4.x : (r == 1 ? t4.y : t4.z);
-                const float p = pred[j];
-                const float den = p * p + 0.01f;
-                const float d = p - t;
-                lsum += d * d / den * inv_n;
-                gp = 2.f * d / den * inv_n;
-            }
-            g[0][s * kRow + r] = f2bf(gp);
+    for (int tile0 = blockIdx.x * 64; tile0 < n; tile0 += gridDim.x * 64) { // workgroup-uniform
+        // the tile's feature rows (zeros beyond n), 16 B per thread and step
+        for (int i = threadIdx.x; i < 64 * 8; i += 64 * kTrainWaves) {
+            const int s = i >> 3, c = (i & 7) * 8;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (tile0 + s < n) v = *reinterpret_cast<const uint4 *>(rec_row(feats, idx, tile0 + s) + c);
+            *reinterpret_cast<uint4 *>(act[0] + s * kRow + c) = v;
         }
-    }
-    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
-    if (l == 0) atomicAdd(loss, lsum);
-    __syncthreads();
-    // backward, last layer first: go = g[cur] is dL/d(output of layer ly)
-    int cur = 0;
-    for (int ly = 4; ly >= 0; ly--) {
-        const uint16_t *x = act[ly];
-        const uint16_t *go = g[cur];
-        const int NO = ly == 4 ? 1 : 4; // output blocks of 16 (the last layer's 3 outputs sit in block 0)
-        const int KO = ly == 4 ? 1 : 2; // K steps of 32 over the outputs
-        // dW[in][out] = sum_s x[s][in] * go[s][out]  (M = in: block w, N = out, K = samples)
-        {
+        __syncthreads();
+        // hidden layers: wave w computes outputs w * 16 .. w * 16 + 15 of all 64 samples
+#pragma nounroll
+        for (int ly = 0; ly < 4; ly++) {
+            const uint16_t *x = act[ly];
+            const uint16_t *W = wt + ly * 4096;
             f32x4 acc[4];
-            for (int nt = 0; nt < 4; nt++) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                const bf16x8 a = gather8(x + (ks * 32 + q * 8) * kRow + w * 16 + r, kRow);
-                for (int nt = 0; nt < NO; nt++)
-                    acc[nt] = mfma(a, gather8(go + (ks * 32 + q * 8) * kRow + nt * 16 + r, kRow), acc[nt]);
+                const bf16x8 b = ld8(W + (w * 16 + r) * 64 + ks * 32 + q * 8);
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+                    acc[mt] = mfma(ld8_lds(x + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
             }
-            for (int nt = 0; nt < NO; nt++)
+            uint16_t *y = act[ly + 1];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int in = w * 16 + q * 4 + j, out = nt * 16 + r;
-                    if (ly == 4 && out >= 3) continue;
-                    atomicAdd(grad + ly * 4096 + in * 64 + out, acc[nt][j]);
-                }
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    y[(mt * 16 + q * 4 + j) * kRow + w * 16 + r] = bf16_one(fmaxf(acc[mt][j], 0.f));
+            __syncthreads();
         }
-        if (ly == 0) break;
-        // gin[s][in] = (sum_o go[s][o] * W[in][o]) * (x[s][in] > 0)  (M = samples, N = in: block w, K = out)
-        uint16_t *gi = g[cur ^ 1];
-        const uint16_t *wl = wb + ly * 4096;
-        f32x4 acc[4];
+        // output layer (outputs 0..2 in block 0): wave w computes samples w * 16 .. w * 16 + 15
+        f32x4 pred = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < KO; ks++) {
-            const bf16x8 b = ld8(wl + (w * 16 + r) * 64 + ks * 32 + q * 8);
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) acc[mt] = mfma(ld8_lds(go + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
-        }
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++)
+        for (int ks = 0; ks < 2; ks++)
+            pred = mfma(ld8_lds(act[4] + (w * 16 + r) * kRow + ks * 32 + q * 8),
+                        ld8(wt + 4 * 4096 + r * 64 + ks * 32 + q * 8), pred);
+        // loss gradient of the output (columns 0..2), written as BF16 into g[0][s][0..31]
+        // (columns 3..31 zero: the last layer's K step for the backward pass is 32)
+        for (int i = threadIdx.x; i < 64 * 32; i += 64 * kTrainWaves) g[0][(i >> 5) * kRow + (i & 31)] = 0;
+        __syncthreads();
+        if (r < 3) {
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int s = mt * 16 + q * 4 + j, in = w * 16 + r;
-                const float relu_on = bf2f(x[s * kRow + in]) > 0.f ? 1.f : 0.f;
-                gi[s * kRow + in] = f2bf(acc[mt][j] * relu_on);
+                const int s = w * 16 + q * 4 + j;
+                float gp = 0.f;
+                if (tile0 + s < n) {
+                    const float4 t4 = target[tile0 + s];
+                    const float t = r == 0 ? t4.x : (r == 1 ? t4.y : t4.z);
+                    const float p = pred[j];
+                    const float den = p * p + 0.01f;
+                    const float d = p - t;
+                    lsum += d * d / den * inv_n;
+                    gp = 2.f * d / den * inv_n;
+                }
+                g[0][s * kRow + r] = bf16_one(gp);
             }
+        }
         __syncthreads();
-        cur ^= 1;
+        // backward, last layer first: go = g[cur] is dL/d(output of layer ly)
+        int cur = 0;
+#pragma nounroll
+        for (int ly = 4; ly >= 0; ly--) {
+            const uint16_t *x = act[ly];
+            const uint16_t *go = g[cur];
+            const int NO = ly == 4 ? 1 : 4; // output blocks of 16 (the last layer's 3 outputs sit in block 0)
+            const int KO = ly == 4 ? 1 : 2; // K steps of 32 over the outputs
+            // dW[in][out] = sum_s x[s][in] * go[s][out]  (M = in: block w, N = out, K = samples)
+            {
+                f32x4 dw[4];
+#pragma unroll
+                for (int nt = 0; nt < 4; nt++) dw[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) {
+                    const bf16x8 a = gather8(x + (ks * 32 + q * 8) * kRow + w * 16 + r, kRow);
+#pragma unroll
+                    for (int nt = 0; nt < NO; nt++)
+                        dw[nt] = mfma(a, gather8(go + (ks * 32 + q * 8) * kRow + nt * 16 + r, kRow), dw[nt]);
+                }
+                const bool first = tile0 == (int)blockIdx.x * 64;
+#pragma unroll
+                for (int nt = 0; nt < NO; nt++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int in = w * 16 + q * 4 + j, out = nt * 16 + r;
+                        if (ly == 4 && out >= 3) continue;
+                        float *d = dst + ly * 4096 + in * 64 + out;
+                        *d = first ? dw[nt][j] : *d + dw[nt][j];
+                    }
+            }
+            if (ly == 0) break;
+            // gin[s][in] = (sum_o go[s][o] * W[in][o]) * (x[s][in] > 0)  (M = samples, N = in: block w, K = out)
+            uint16_t *gi = g[cur ^ 1];
+            const uint16_t *wl = wb + ly * 4096;
+            f32x4 acc[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < KO; ks++) {
+                const bf16x8 b = ld8(wl + (w * 16 + r) * 64 + ks * 32 + q * 8);
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+                    acc[mt] = mfma(ld8_lds(go + (mt * 16 + r) * kRow + ks * 32 + q * 8), b, acc[mt]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int s = mt * 16 + q * 4 + j, in = w * 16 + r;
+                    const float relu_on = bf2f(x[s * kRow + in]) > 0.f ? 1.f : 0.f;
+                    gi[s * kRow + in] = bf16_one(acc[mt][j] * relu_on);
+                }
+            __syncthreads();
+            cur ^= 1;
+        }
+        __syncthreads(); // the next tile overwrites act and g
     }
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
+    if (l == 0 && lsum != 0.f) atomicAdd(loss, lsum);
+    if (blockIdx.x * 64 >= n) // no tile: a zero gradient
+        for (int i = threadIdx.x; i < kNrcParams; i += 64 * kTrainWaves) dst[i] = 0.f;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master weights, then the two BF16 copies.
-// grad is read and zeroed.  t: step number (1-based).
-__global__ void k_nrc_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
-                           float *__restrict__ grad, uint16_t *__restrict__ wt, uint16_t *__restrict__ wb, float lr,
-                           float b1, float b2, float eps, float bc1, float bc2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 5 * 4096) return;
+// The gradient is the sum of the training kernel's `groups` partials, in a fixed
+// order: a workgroup of kAdamSplit x 64 threads takes 64 consecutive weights,
+// wavefront r sums partials r, r + kAdamSplit, .. (64 consecutive floats per
+// load), and the kAdamSplit sums are added in LDS.  The sum is also stored in
+// grad (read by the test hooks).  t: step number (1-based).
+static constexpr int kAdamSplit = 4;
+__global__ __launch_bounds__(64 * kAdamSplit) void k_nrc_adam(float *__restrict__ w, float *__restrict__ m,
+                                                               float *__restrict__ v, const float *__restrict__ partial,
+                                                               int groups, float *__restrict__ grad,
+                                                               uint16_t *__restrict__ wt, uint16_t *__restrict__ wb,
+                                                               float lr, float b1, float b2, float eps, float bc1,
+                                                               float bc2) {
+    __shared__ float part[kAdamSplit][64];
+    const int lane = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane; // 5 * 4096 is a multiple of 64
+    float gr = 0.f;
+#pragma unroll 8
+    for (int k = r; k < groups; k += kAdamSplit) gr += partial[(size_t)k * kNrcParams + i];
+    part[r][lane] = gr;
+    __syncthreads();
+    if (r != 0) return;
+    gr = 0.f;
+#pragma unroll
+    for (int k = 0; k < kAdamSplit; k++) gr += part[k][lane];
     int ly = i >> 12, in = (i >> 6) & 63, out = i & 63;
-    float gr = grad[i];
-    grad[i] = 0.f;
+    if (ly == 4 && out >= 3) gr = 0.f; // padding columns (not written by the training kernel)
+    grad[i] = gr;
     float mi = b1 * m[i] + (1.f - b1) * gr;
     float vi = b2 * v[i] + (1.f - b2) * gr * gr;
     m[i] = mi;
@@ -425,18 +462,18 @@ void nrc_launch_infer(const uint16_t *feats, const int32_t *idx, const uint32_t 
     const int blocks = (waves + 3) / 4;
     hipLaunchKernelGGL(k_nrc_infer<kInferTiles>, dim3(blocks), dim3(256), 0, st, feats, idx, n_dev, n_max, wt, out);
 }
+int nrc_train_groups(int n_max) { return std::max(1, std::min((n_max + 63) / 64, kNrcMaxTrainGroups)); }
 void nrc_launch_train(const uint16_t *feats, const int32_t *idx, const float4 *target, const uint32_t *n_dev,
                       int cap, int base, int n_max, const NrcWeights &W, hipStream_t st) {
     if (n_max <= 0) return;
-    hipLaunchKernelGGL(k_nrc_train, dim3((n_max + 63) / 64), dim3(64 * kTrainWaves), 0, st, feats, idx, target, n_dev,
-                       cap, base,
-                       n_max, W.wt, W.wb, W.grad, W.loss);
+    hipLaunchKernelGGL(k_nrc_train, dim3(nrc_train_groups(n_max)), dim3(64 * kTrainWaves), 0, st, feats, idx, target,
+                       n_dev, cap, base, n_max, W.wt, W.wb, W.partial, W.loss);
 }
-void nrc_launch_adam(const NrcWeights &W, float lr, int step, hipStream_t st) {
+void nrc_launch_adam(const NrcWeights &W, int n_max, float lr, int step, hipStream_t st) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
-    hipLaunchKernelGGL(k_nrc_adam, dim3((5 * 4096 + 255) / 256), dim3(256), 0, st, W.w, W.m, W.v, W.grad, W.wt, W.wb,
-                       lr, b1, b2, eps, bc1, bc2);
+    hipLaunchKernelGGL(k_nrc_adam, dim3(kNrcParams / 64), dim3(64 * kAdamSplit), 0, st, W.w, W.m, W.v, W.partial,
+                       nrc_train_groups(n_max), W.grad, W.wt, W.wb, lr, b1, b2, eps, bc1, bc2);
 }
 
 } // namespace tpt
