@@ -14,9 +14,9 @@ it), the CPU oracle timed on a bounded sample, and the radiance-cache rows.
 Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
                       16-256 spp) through PathTracer.render_batch: up to three
-                      path pools on as many streams, a captured pipeline wave
-                      per pool replayed, autotuned queue capacity and
-                      workgroup size;
+                      path pools on as many streams, the whole batch captured
+                      in one hipGraph and launched once, autotuned queue
+                      capacity and workgroup size;
                       per-scene completion times and the same scenes rendered
                       one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
@@ -380,11 +380,15 @@ def bench_batch(args):
     out = base_line(paths / (ms_per_step / 1e3) / 1e6, ms_per_step, args, {
         "workload": (f"batch_{len(devs)}_rooms_{ranges['tris'][0] // 1000}k-{ranges['tris'][1] // 1000}k_tris_"
                      f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8"),
-        "tracer": "plain HIP wavefront, three path pools on three streams, captured pipeline wave per pool replayed",
+        "tracer": ("plain HIP wavefront, three path pools on three streams; scenes under 2^23 paths share batched "
+                   "pipeline waves, larger ones run plain pipeline waves of their own; the whole batch captured in "
+                   "one hipGraph and launched once" if st.get("graphs") else
+                   "plain HIP wavefront, three path pools on three streams, captured pipeline wave per pool replayed"),
         "paths_per_step": paths,
         "queue_capacity": cap,
         "block_size": block,
         "waves": st["waves"],
+        "one_graph_launch": bool(st.get("graphs")),
         "tune_ms": round(st["tune_ms"], 1),
         "triangles": [min(sp["tris"] for sp in specs), max(sp["tris"] for sp in specs)],
     })

@@ -249,15 +249,17 @@ typedef struct {
     uint64_t paths;         /* total paths of the batch */
     int32_t queue_capacity; /* path slots per pipeline wave (tuned or given) */
     int32_t block_size;     /* threads per workgroup of generate / shade (tuned or given) */
-    int32_t waves;          /* pipeline waves: replays of the captured hipGraph + plain waves of large scenes */
-    int32_t pad_;
+    int32_t waves;          /* pipeline waves: batched waves + plain waves of large scenes */
+    int32_t graphs;         /* 1: the render ran as one captured hipGraph launch; 0: per-wave graph replays */
 } tpt_batch_stats;
 /* Renders n scenes (all created on the current device with the same node layout, n
- * <= 1024) through one pipeline: a pipeline wave (generate, then max_depth + 1 x
- * extend / shade / shadow) is captured once per path pool in a hipGraph and
- * replayed until all paths are done, alternating between up to three pools on as
- * many streams (one with TPT_STREAMS=1); each wave's accumulation follows the
- * previous wave's, so the images are bit-identical to single-scene renders.
+ * <= 1024) through one pipeline: pipeline waves (generate, then max_depth + 1 x
+ * extend / shade / shadow) of up to queue_capacity paths, mixing scenes,
+ * alternate between up to three path pools on as many streams (one with
+ * TPT_STREAMS=1); each wave's accumulation follows the previous wave's, so the
+ * images are bit-identical to single-scene renders.  The whole render is
+ * captured into one hipGraph and launched once (TPT_BATCH_GRAPH=0: one captured
+ * wave per pool, replayed from the host).
  * Scenes of at least 2^23 paths (TPT_BATCH_LARGE=10..40 sets the exponent) fill
  * the GPU alone: they run after the batched ones as plain pipeline waves of their
  * own (tpt_hip_render's kernels and 2^24-slot waves) on the same pools.

@@ -121,3 +121,24 @@ def test_batch_large_scenes_as_plain_waves(hip, batch, monkeypatch, large_log2, 
         assert len(st["scene_ms"]) == len(devs) and all(t > 0 for t in st["scene_ms"])
         for i, (a, b) in enumerate(zip(imgs, ref)):
             assert np.array_equal(a.raw(), b), f"scene {i} differs (2^{large_log2}, capacity {cap})"
+
+
+@pytest.mark.parametrize("graph", ["1", "0"])
+def test_batch_one_graph_or_wave_replays(hip, batch, monkeypatch, graph):
+    # by default the whole batch (batched waves, large scenes' plain waves,
+    # finalize) is captured into one hipGraph and launched once; TPT_BATCH_GRAPH=0
+    # replays one captured wave per pool from the host; same images either way
+    hosts, devs, spps, seeds = batch
+    monkeypatch.setenv("TPT_BATCH_GRAPH", graph)
+    monkeypatch.setenv("TPT_BATCH_LARGE", "13")
+    kw = dict(max_depth=6, rr_depth=3, hdr=True)
+    ref = singles(hip, devs, spps, seeds, **kw)
+    pt = hip.PathTracer("hip")
+    for _ in range(2):  # twice: nothing of one call's graph outlives it
+        imgs = pt.render_batch(devs, spps, seeds, queue_capacity=4096, block_size=128, **kw)
+        st = pt.last_batch_stats()
+        assert st["graphs"] == int(graph)
+        times = st["scene_ms"]
+        assert len(times) == len(devs) and all(t > 0 for t in times)
+        for i, (a, b) in enumerate(zip(imgs, ref)):
+            assert np.array_equal(a.raw(), b), f"scene {i} differs (one graph {graph})"

@@ -38,6 +38,8 @@ void launch_batch_wave(const DBatch &, const DPaths &, int, int, int, int, bool,
                        hipStream_t);
 void launch_batch_tail(const DBatch &, const DPaths &, uint64_t, hipStream_t);
 void launch_batch_zero(const DBatch &, const DPaths &, int, uint64_t, hipStream_t);
+void launch_batch_state(const DBatch &, uint64_t, uint64_t, hipStream_t);
+void launch_stamp(uint64_t *, hipStream_t);
 void launch_batch_finalize(const DBatch &, const DPaths &, float *, uint64_t, hipStream_t);
 void launch_cache_reset(const DCache &, hipStream_t);
 void launch_cache_apply(const DPaths &, const DCache &, const float4 *, int, hipStream_t);
@@ -396,6 +398,27 @@ void ensure_paths(tpt_hip_scene *h, size_t slots, int pixels) {
 void ensure_stream2(tpt_hip_scene *h) {
     if (!h->stream2) HIP_OK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
 }
+
+// Device memory owned by a call (freed on every exit).
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    explicit DeviceBuffer(size_t bytes) { HIP_OK(hipMalloc(&ptr, bytes)); }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+// An executable graph owned by a call.
+struct GraphExec {
+    hipGraphExec_t exec = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec &) = delete;
+    GraphExec &operator=(const GraphExec &) = delete;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
 
 // HIP events owned by a render: destroyed on every exit, errors included.
 struct EventVec : std::vector<hipEvent_t> {
@@ -908,14 +931,25 @@ struct BatchRun {
         pools = k + 1;
         for (int q = 0; q < pools; q++) batch(q).pools = pools;
     }
+    int block = 256; // generate / shade workgroup size of the pipeline waves
+    // the pipeline part of one wave (every bounce's kernels) on stream s
+    void launch_wave(int k, hipStream_t s) const {
+        launch_batch_wave(xB_or_B(k), pbs[k]->P, max_depth, layout, block, trav_block, lds_stack, mode, chunk, max_lds,
+                          num_cu, total_pixels, s);
+    }
+    const DBatch &xB_or_B(int k) const { return k ? xB[k - 1] : B; }
+    // waves of capacity `cap` with workgroup size `blk` (sets every pool's view)
+    void shape(int cap, int blk) {
+        block = blk;
+        for (int k = 0; k < pools; k++) batch(k).cap = cap;
+    }
     // capture the pipeline part of one wave of (cap, block), per pool, into executable graphs
     void capture(int cap, int block, hipGraphExec_t ge[kMaxPools]) {
-        for (int k = 0; k < pools; k++) batch(k).cap = cap;
+        shape(cap, block);
         for (int k = 0; k < pools; k++) {
             hipGraph_t g = nullptr;
             HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_batch_wave(batch(k), pool(k).P, max_depth, layout, block, trav_block, lds_stack, mode, chunk,
-                              max_lds, num_cu, total_pixels, st);
+            launch_wave(k, st);
             HIP_OK(hipStreamEndCapture(st, &g));
             hipError_t e = hipGraphInstantiate(&ge[k], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -934,9 +968,17 @@ struct BatchRun {
         launch_batch_zero(B, pbs[0]->P, fp64, total_pixels, st);
     }
     int waves(int cap) const { return (int)((total_paths + (uint64_t)cap - 1) / (uint64_t)cap); }
-    // enqueue waves [0, W): wave w's pipeline graph on pool w % pools's stream, then
-    // its tail after the previous wave's tail; done[w] (if given) is recorded after wave w
-    void run(hipGraphExec_t ge[kMaxPools], int W, hipEvent_t *done) {
+    // the reset of reset() as kernels (for a captured graph): pool k's first wave
+    // starts at start + k * cap, accumulators zeroed
+    void reset_on_device(int cap, uint64_t start = 0) {
+        for (int k = 0; k < pools; k++) launch_batch_state(batch(k), start + (uint64_t)k * (uint64_t)cap, total_paths, st);
+        launch_batch_zero(B, pbs[0]->P, fp64, total_pixels, st);
+    }
+    // enqueue waves [0, W): wave w's pipeline (its per-pool graph ge[k], or the
+    // kernels themselves when ge is null, e.g. while st is being captured) on pool
+    // w % pools's stream, then its tail after the previous wave's tail; stamps[w]
+    // (if given) gets the device clock after wave w's tail
+    void run(hipGraphExec_t *ge, int W, uint64_t *stamps) {
         std::vector<hipEvent_t> tail(W);
         for (auto &e : tail) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         if (pools > 1) { // the other pools start after the reset on st
@@ -949,11 +991,12 @@ struct BatchRun {
         for (int w = 0; w < W; w++) {
             const int k = w % pools;
             hipStream_t s = stream(k);
-            HIP_OK(hipGraphLaunch(ge[k], s));
+            if (ge) HIP_OK(hipGraphLaunch(ge[k], s));
+            else launch_wave(k, s);
             if (pools > 1 && w > 0) HIP_OK(hipStreamWaitEvent(s, tail[w - 1], 0));
             launch_batch_tail(batch(k), pool(k).P, total_pixels, s);
+            if (stamps) launch_stamp(stamps + w, s);
             HIP_OK(hipEventRecord(tail[w], s));
-            if (done) HIP_OK(hipEventRecord(done[w], s));
         }
         if (pools > 1 && W > 0) HIP_OK(hipStreamWaitEvent(st, tail[W - 1], 0)); // st continues after the last wave
         for (auto &e : tail) HIP_OK(hipEventDestroy(e));
@@ -1165,78 +1208,105 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             }
             tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
         }
-        // the render: one captured pipeline wave per pool, replayed until every path
-        // of the batched scenes is done; then the large scenes' plain waves
+        // The render: the batched scenes' pipeline waves, then the large scenes'
+        // plain waves, over the pools.  By default the whole render is captured
+        // from the pools' streams into ONE hipGraph and launched once
+        // (TPT_BATCH_GRAPH=0: the per-pool wave graphs are replayed from the host
+        // instead).  Completion times come from device clock stamps: [0] the
+        // start, [1 + w] after batched wave w, [1 + W + i] after large scene i.
         auto tr = std::chrono::steady_clock::now();
+        const bool one_graph = env_int("TPT_BATCH_GRAPH", 1, 0, 1) != 0;
+        int plain_waves = 0;
         hipGraphExec_t ge[BatchRun::kMaxPools] = {};
         const int W = nb > 0 ? run.waves(best_cap) : 0;
-        EventVec ev(W + 1);
-        for (auto &e : ev) HIP_OK(hipEventCreate(&e));
-        if (nb > 0) {
+        DeviceBuffer stamp_buf((size_t)(2 + W + n) * 8);
+        uint64_t *stamps = (uint64_t *)stamp_buf.ptr;
+        if (nb > 0 && !one_graph) {
             run.capture(best_cap, best_block, ge);
             run.reset(best_cap);
-            HIP_OK(hipEventRecord(ev[0], run.st));
-            run.run(ge, W, ev.data() + 1);
-            launch_batch_finalize(B, run.pbs[0]->P, run.pbs[0]->out, run.total_pixels, run.st);
-        } else {
-            HIP_OK(hipEventRecord(ev[0], run.st));
         }
-        // large scenes: plain pipeline waves on the same pools, continuing the
-        // rotation; each scene's sums and image are its range of the pools' arrays
-        EventVec pev;
-        std::vector<int> done_ev(n, -1);
-        int plain_waves = 0;
-        if (nb < n) {
-            PlainWaves pw;
-            pw.npools = run.pools;
-            pw.block = 256;
-            pw.cus = run.num_cu;
-            for (int k = 0; k < run.pools; k++) pw.streams[k] = run.stream(k);
-            if (run.pools > 1) { // every pool's plain waves follow what is queued on st so far
-                const int e = PlainWaves::record(pev, run.st, false);
-                for (int k = 1; k < run.pools; k++) HIP_OK(hipStreamWaitEvent(run.stream(k), pev[e], 0));
-            }
-            int lane0 = W % run.pools;
-            for (int i = nb; i < n; i++) {
-                const size_t o3 = 3 * run.pix_off[i];
-                for (int k = 0; k < run.pools; k++) {
-                    DPaths P = run.pool(k).P;
-                    P.sub_f += o3, P.pix_f += o3, P.sub_d += o3, P.pix_d += o3;
-                    pw.pools[k] = P;
+        hipGraph_t graph = nullptr;
+        GraphExec whole;
+        if (one_graph) HIP_OK(hipStreamBeginCapture(run.st, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_stamp(stamps, run.st);
+            if (nb > 0) {
+                if (one_graph) {
+                    run.shape(best_cap, best_block);
+                    run.reset_on_device(best_cap);
                 }
-                int prev = -1;
-                const int last = pw.enqueue(scenes[i]->S, rend[i], rend[i].k, seeds[i], lane0, pev, prev);
-                // the last accumulate runs on the last wave's stream after all earlier ones
-                launch_finalize(rend[i], pw.pools[last], run.pbs[0]->out + o3, run.stream(last));
-                done_ev[i] = PlainWaves::record(pev, run.stream(last), true);
-                lane0 = (last + 1) % run.pools;
+                run.run(one_graph ? nullptr : ge, W, stamps + 1);
+                launch_batch_finalize(B, run.pbs[0]->P, run.pbs[0]->out, run.total_pixels, run.st);
             }
-            plain_waves = pw.waves;
-            for (int k = 0; k < run.pools; k++) { // st continues after every pool's last plain wave
-                if (run.stream(k) == run.st) continue;
-                const int e = PlainWaves::record(pev, run.stream(k), false);
-                HIP_OK(hipStreamWaitEvent(run.st, pev[e], 0));
+            // large scenes: plain pipeline waves on the same pools, continuing the
+            // rotation; each scene's sums and image are its range of the pools' arrays
+            EventVec pev;
+            if (nb < n) {
+                PlainWaves pw;
+                pw.npools = run.pools;
+                pw.block = 256;
+                pw.cus = run.num_cu;
+                for (int k = 0; k < run.pools; k++) pw.streams[k] = run.stream(k);
+                if (run.pools > 1) { // every pool's plain waves follow what is queued on st so far
+                    const int e = PlainWaves::record(pev, run.st, false);
+                    for (int k = 1; k < run.pools; k++) HIP_OK(hipStreamWaitEvent(run.stream(k), pev[e], 0));
+                }
+                int lane0 = W % run.pools;
+                for (int i = nb; i < n; i++) {
+                    const size_t o3 = 3 * run.pix_off[i];
+                    for (int k = 0; k < run.pools; k++) {
+                        DPaths P = run.pool(k).P;
+                        P.sub_f += o3, P.pix_f += o3, P.sub_d += o3, P.pix_d += o3;
+                        pw.pools[k] = P;
+                    }
+                    int prev = -1;
+                    const int last = pw.enqueue(scenes[i]->S, rend[i], rend[i].k, seeds[i], lane0, pev, prev);
+                    // the last accumulate runs on the last wave's stream after all earlier ones
+                    launch_finalize(rend[i], pw.pools[last], run.pbs[0]->out + o3, run.stream(last));
+                    launch_stamp(stamps + 1 + W + i, run.stream(last));
+                    lane0 = (last + 1) % run.pools;
+                }
+                plain_waves = pw.waves;
+                for (int k = 0; k < run.pools; k++) { // st continues after every pool's last plain wave
+                    if (run.stream(k) == run.st) continue;
+                    const int e = PlainWaves::record(pev, run.stream(k), false);
+                    HIP_OK(hipStreamWaitEvent(run.st, pev[e], 0));
+                }
             }
+            HIP_OK(hipGetLastError());
+        } catch (...) {
+            if (one_graph) { // end the capture before the error leaves
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(run.st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+            }
+            throw;
         }
-        HIP_OK(hipGetLastError());
+        if (one_graph) {
+            HIP_OK(hipStreamEndCapture(run.st, &graph));
+            hipError_t e = hipGraphInstantiate(&whole.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_OK(e);
+            HIP_OK(hipGraphLaunch(whole.exec, run.st));
+        }
         for (int i = 0; i < n; i++)
             HIP_OK(hipMemcpyAsync(out_rgb[i], run.pbs[0]->out + 3 * run.pix_off[i], (run.pix_off[i + 1] - run.pix_off[i]) * 12,
                                   hipMemcpyDeviceToHost, run.st));
         HIP_OK(hipStreamSynchronize(run.st));
         auto t1 = std::chrono::steady_clock::now();
         if (scene_ms) {
+            std::vector<uint64_t> ts((size_t)(1 + W + n));
+            HIP_OK(hipMemcpy(ts.data(), stamps, ts.size() * 8, hipMemcpyDeviceToHost));
+            int khz = 100000;
+            if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, scenes[0]->device) != hipSuccess || khz <= 0)
+                khz = 100000;
             for (int i = 0; i < n; i++) {
-                float ms = 0;
-                if (i < nb) {
-                    int w = (int)((run.path_off[i + 1] - 1) / (uint64_t)best_cap);
-                    HIP_OK(hipEventElapsedTime(&ms, ev[0], ev[w + 1]));
-                } else {
-                    HIP_OK(hipEventElapsedTime(&ms, ev[0], pev[done_ev[i]]));
-                }
-                scene_ms[order[i]] = ms;
+                const uint64_t t = i < nb ? ts[1 + (run.path_off[i + 1] - 1) / (uint64_t)best_cap] : ts[1 + W + i];
+                scene_ms[order[i]] = (double)(t - ts[0]) / (double)khz;
             }
         }
-        for (int k = 0; k < run.pools; k++) // none captured when every scene ran as plain waves
+        for (int k = 0; k < run.pools; k++) // none captured with one graph or no batched scene
             if (ge[k]) (void)hipGraphExecDestroy(ge[k]);
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
@@ -1246,6 +1316,7 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             stats->queue_capacity = best_cap;
             stats->block_size = best_block;
             stats->waves = W + plain_waves;
+            stats->graphs = one_graph ? 1 : 0;
         }
         (void)t0;
         return TPT_OK;

@@ -1677,6 +1677,20 @@ __global__ void k_accumulate_batch(DBatch B, DPaths P) {
     accum_run(R, P, q, (int)s_lo, (int)(s_hi - s_lo + 1), [&](int j) { return slot0 + (s_lo + j) * px; }, false);
 }
 
+// Start of a pool's first pipeline wave (`g0`) and the batch's path count.
+__global__ void k_batch_state(DBatch B, uint64_t g0, uint64_t total) {
+    if (threadIdx.x == 0) {
+        B.state[0] = g0;
+        B.state[1] = total;
+    }
+}
+
+// Device wall-clock stamp (constant rate, hipDeviceAttributeWallClockRate) of the
+// point in its stream where it runs: completion times inside a captured graph.
+__global__ void k_stamp(uint64_t *dst) {
+    if (threadIdx.x == 0) *dst = (uint64_t)wall_clock64();
+}
+
 __global__ void k_batch_end(DBatch B) {
     if (threadIdx.x == 0) B.state[0] += (uint64_t)B.cap * (uint64_t)B.pools;
 }
@@ -2026,6 +2040,10 @@ void launch_batch_tail(const DBatch &B, const DPaths &P, uint64_t total_pixels, 
     hipLaunchKernelGGL(k_accumulate_batch, dim3((unsigned)((total_pixels + 255) / 256)), dim3(256), 0, st, B, P);
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, B);
 }
+void launch_batch_state(const DBatch &B, uint64_t g0, uint64_t total, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_state, dim3(1), dim3(64), 0, st, B, g0, total);
+}
+void launch_stamp(uint64_t *dst, hipStream_t st) { hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, st, dst); }
 void launch_batch_zero(const DBatch &B, const DPaths &P, int fp64, uint64_t total_pixels, hipStream_t st) {
     hipLaunchKernelGGL(k_batch_zero, dim3((unsigned)((3 * total_pixels + 255) / 256)), dim3(256), 0, st, B, P, fp64);
 }

@@ -198,6 +198,7 @@ PYBIND11_MODULE(_C, m) {
             d["queue_capacity"] = s.queue_capacity;
             d["block_size"] = s.block_size;
             d["waves"] = s.waves;
+            d["graphs"] = s.graphs;
             d["scene_ms"] = self.last_batch_scene_ms();
             return d;
         });
