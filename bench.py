@@ -126,6 +126,24 @@ def cpu_baseline(spp=64, rows=256, width=1024, height=768):
     }
 
 
+def oracle_sample(host, height, spp, max_depth, rr_depth, y0, y1, what):
+    """The CPU oracle on rows [y0, y1) of a host scene (a bounded sample of a workload)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    model, threads = host_cpu()
+    oracle = O.OracleScene(host)
+    p = O.params(spp, max_depth=max_depth, rr_depth=rr_depth, seed=1)
+    t0 = time.perf_counter()
+    oracle.render(height, p, y0, y1, threads=threads)
+    dt = time.perf_counter() - t0
+    width = host.camera().width
+    paths = width * (y1 - y0) * spp
+    return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: rows {y0}-{y1} of {width}x{height} at {spp} spp ({paths} paths, {dt:.1f} s)",
+            "cpu": model, "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,8 +410,10 @@ def bench_batch(args):
         "tune_ms": round(st["tune_ms"], 1),
         "triangles": [min(sp["tris"] for sp in specs), max(sp["tris"] for sp in specs)],
     })
-    out["roofline"] = None
-    out["cpu_baseline"] = None
+    out["roofline"] = None  # the batch's kernels run captured in one graph: no per-kernel time in this process
+    out["cpu_baseline"] = None if args.no_cpu else oracle_sample(
+        hosts[0], specs[0]["height"], specs[0]["spp"], c2["max_depth"], c2["rr_depth"], 0, specs[0]["height"],
+        f"the batch's first scene ({specs[0]['tris']} triangles), whole image")
     out["scene_completion_ms"] = {"min": round(done[0], 1), "median": round(done[len(done) // 2], 1),
                                   "p90": round(done[int(0.9 * (len(done) - 1))], 1), "max": round(done[-1], 1)}
     # the same batch with the capacity tuned up to 2^24 slots (above the configuration's
@@ -433,7 +453,8 @@ def cache_comparison(args, frame_counts, width=1920, height=1080):
 
     c3 = scenes.CONFIG3
     W, H = width, height
-    dev = scenes.textured_room(W, H).to("hip")
+    host = scenes.textured_room(W, H)
+    dev = host.to("hip")
     pt = tinypt_amd.PathTracer("hip")
     kw = dict(max_depth=c3["max_depth"], rr_depth=c3["rr_depth"])
     t = time.perf_counter()
@@ -468,8 +489,11 @@ def cache_comparison(args, frame_counts, width=1920, height=1080):
                            mean_bias=round(float(img.mean() / ref.mean() - 1.0), 4)))
         print(f"# cached {frames:4d} frames: {dt * 1e3:8.1f} ms relMSE {e:.5f} -> equal-error speedup "
               f"{cached[-1]['speedup']:.2f} (bias {cached[-1]['mean_bias']:+.4f})", file=sys.stderr, flush=True)
+    mlp = mlp_roofline(host, kw, args.cache_suffix)
+    print(f"# MLP kernels alone: {mlp}", file=sys.stderr, flush=True)
     return {
         "workload": f"textured_room_264612tri_{W}x{H}_depth12_rgba8_alpha_bump",
+        "roofline": mlp,
         "reference": f"plain HIP tracer, {args.ref_spp} spp, FP64 accumulation ({ref_s:.1f} s)",
         "cache": ("5x64 BF16 MFMA MLP, Adam 1e-3, 2^16 suffixes/frame, 8 steps/frame, cache after 2 bounces, "
                   f"training suffixes end in the cache after {args.cache_suffix} more vertices"),
@@ -479,6 +503,46 @@ def cache_comparison(args, frame_counts, width=1920, height=1080):
         # the row of the most frames: the longest, least noisy comparison
         "equal_relmse_speedup": cached[-1]["speedup"],
         "equal_relmse_speedup_frames": cached[-1]["frames"],
+    }
+
+
+NRC_FLOP_PER_QUERY = 4 * 64 * 64 * 2 + 64 * 3 * 2  # four 64x64 hidden layers + the 3-output layer
+NRC_BYTES_PER_QUERY = 128 + 4 + 16  # 64 BF16 inputs, the record's row index, the float4 prediction
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def mlp_roofline(host, kw, suffix, frames=16):
+    """Roofline of the cache's inference kernel (k_nrc_infer) alone: a cached render on a
+    one-lane scene handle, where the library's HIP events around each frame's inference
+    and training see no other kernel beside them."""
+    import tinypt_amd
+
+    prev = os.environ.get("TPT_STREAMS")
+    os.environ["TPT_STREAMS"] = "1"
+    try:
+        dev1 = host.to("hip")
+    finally:
+        if prev is None:
+            os.environ.pop("TPT_STREAMS")
+        else:
+            os.environ["TPT_STREAMS"] = prev
+    tinypt_amd.render_cached(dev1, 2, cache_depth=2, reset=True, seed=3, suffix_len=suffix, **kw)  # warm-up
+    _, st = tinypt_amd.render_cached(dev1, frames, cache_depth=2, train_records=1 << 16, learning_rate=1e-3,
+                                     reset=True, weight_seed=5, seed=1, suffix_len=suffix, **kw)
+    del dev1
+    infer_s = st["infer_ms"] / frames / 1e3
+    q = st["queries"]
+    tflops = q * NRC_FLOP_PER_QUERY / infer_s / 1e12
+    gbs = q * NRC_BYTES_PER_QUERY / infer_s / 1e9
+    steps = 8 * frames
+    return {
+        "bound": "mfma", "achieved": round(tflops, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None, "kernel": "k_nrc_infer",
+        "avg_launch_ms": round(infer_s * 1e3, 4), "queries_per_launch": q, "flop_per_query": NRC_FLOP_PER_QUERY,
+        "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "bytes_per_query": NRC_BYTES_PER_QUERY},
+        "train_step_ms": round(st["train_ms"] / steps, 4),
+        "source": "one-lane cached render, 16 frames: HIP events around each frame's inference and training",
     }
 
 
@@ -497,7 +561,7 @@ def bench_cache(args):
         "paths_per_step": paths,
         "reference": res["reference"],
     })
-    out["roofline"] = None
+    out["roofline"] = res["roofline"]
     out["cpu_baseline"] = None
     out["equal_relmse_speedup_vs_plain"] = res["equal_relmse_speedup"]
     out["plain_relmse_times_s"] = res["plain_relmse_times_s"]
@@ -584,7 +648,9 @@ def bench_city(args):
     })
     out["roofline"] = roofline
     out.update(extra)
-    out["cpu_baseline"] = None
+    out["cpu_baseline"] = None if args.no_cpu else oracle_sample(
+        host, c["height"], 4, c["max_depth"], c["rr_depth"], c["height"] // 2 - 16, c["height"] // 2 + 16,
+        "the city at 4 spp")
     out["vs_reference"] = rows
     print(json.dumps(out))
 
