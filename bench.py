@@ -14,9 +14,10 @@ it), the CPU oracle timed on a bounded sample, and the radiance-cache rows.
 Other workloads (one JSON line each, same fields):
   --workload batch    64 seeded rooms (2k-500k triangles, 128^2-1024^2 pixels,
                       16-256 spp) through PathTracer.render_batch: up to three
-                      path pools on as many streams, the whole batch captured
-                      in one hipGraph and launched once, autotuned queue
-                      capacity and workgroup size;
+                      path pools on as many streams, small scenes in shared
+                      captured pipeline waves, large ones in plain waves,
+                      autotuned queue capacity and workgroup size; also the
+                      same batch as one hipGraph launch (TPT_BATCH_GRAPH=1);
                       per-scene completion times and the same scenes rendered
                       one by one
   --workload spheres  the analytic two-spheres box, 512x384, 64 spp, depth 10, RR 5
@@ -399,14 +400,13 @@ def bench_batch(args):
         "workload": (f"batch_{len(devs)}_rooms_{ranges['tris'][0] // 1000}k-{ranges['tris'][1] // 1000}k_tris_"
                      f"{ranges['side'][0]}-{ranges['side'][1]}px_{ranges['spp'][0]}-{ranges['spp'][1]}spp_depth8"),
         "tracer": ("plain HIP wavefront, three path pools on three streams; scenes under 2^23 paths share batched "
-                   "pipeline waves, larger ones run plain pipeline waves of their own; the whole batch captured in "
-                   "one hipGraph and launched once" if st.get("graphs") else
-                   "plain HIP wavefront, three path pools on three streams, captured pipeline wave per pool replayed"),
+                   "pipeline waves (one captured wave per pool, replayed), larger ones run plain pipeline waves of "
+                   "their own" + ("; the whole batch captured in one hipGraph and launched once" if st.get("graphs")
+                                  else "")),
         "paths_per_step": paths,
         "queue_capacity": cap,
         "block_size": block,
         "waves": st["waves"],
-        "one_graph_launch": bool(st.get("graphs")),
         "tune_ms": round(st["tune_ms"], 1),
         "triangles": [min(sp["tris"] for sp in specs), max(sp["tris"] for sp in specs)],
     })
@@ -429,6 +429,16 @@ def bench_batch(args):
         os.environ.pop("TPT_BATCH_MAX_CAP")
     out["tuned_up_to_2^24"] = {"mpaths_per_s": round(paths / (ms24 / 1e3) / 1e6, 2), "ms": round(ms24, 1),
                                "queue_capacity": st24["queue_capacity"], "block_size": st24["block_size"]}
+    # the same batch captured into one hipGraph and launched once
+    os.environ["TPT_BATCH_GRAPH"] = "1"
+    try:
+        t = time.perf_counter()
+        pt.render_batch(devs, spps, seeds, queue_capacity=cap, block_size=block, **kw)
+        ms1 = (time.perf_counter() - t) * 1e3
+        g1 = pt.last_batch_stats()["graphs"]
+    finally:
+        os.environ.pop("TPT_BATCH_GRAPH")
+    out["one_graph_launch"] = {"mpaths_per_s": round(paths / (ms1 / 1e3) / 1e6, 2), "ms": round(ms1, 1), "graphs": g1}
     if not args.no_seq:
         t = time.perf_counter()
         for d, n, sd in zip(devs, spps, seeds):

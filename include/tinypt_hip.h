@@ -257,9 +257,9 @@ typedef struct {
  * extend / shade / shadow) of up to queue_capacity paths, mixing scenes,
  * alternate between up to three path pools on as many streams (one with
  * TPT_STREAMS=1); each wave's accumulation follows the previous wave's, so the
- * images are bit-identical to single-scene renders.  The whole render is
- * captured into one hipGraph and launched once (TPT_BATCH_GRAPH=0: one captured
- * wave per pool, replayed from the host).
+ * images are bit-identical to single-scene renders.  One captured wave per pool
+ * is replayed from the host (TPT_BATCH_GRAPH=1: the whole render is captured into
+ * one hipGraph and launched once).
  * Scenes of at least 2^23 paths (TPT_BATCH_LARGE=10..40 sets the exponent) fill
  * the GPU alone: they run after the batched ones as plain pipeline waves of their
  * own (tpt_hip_render's kernels and 2^24-slot waves) on the same pools.

@@ -125,8 +125,8 @@ def test_batch_large_scenes_as_plain_waves(hip, batch, monkeypatch, large_log2, 
 
 @pytest.mark.parametrize("graph", ["1", "0"])
 def test_batch_one_graph_or_wave_replays(hip, batch, monkeypatch, graph):
-    # by default the whole batch (batched waves, large scenes' plain waves,
-    # finalize) is captured into one hipGraph and launched once; TPT_BATCH_GRAPH=0
+    # TPT_BATCH_GRAPH=1 captures the whole batch (batched waves, large scenes'
+    # plain waves, finalize) into one hipGraph launched once; 0 (the default)
     # replays one captured wave per pool from the host; same images either way
     hosts, devs, spps, seeds = batch
     monkeypatch.setenv("TPT_BATCH_GRAPH", graph)

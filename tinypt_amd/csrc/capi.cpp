@@ -1208,14 +1208,17 @@ int tpt_hip_render_batch(tpt_hip_scene *const *scenes_in, const tpt_batch_item *
             }
             tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
         }
-        // The render: the batched scenes' pipeline waves, then the large scenes'
-        // plain waves, over the pools.  By default the whole render is captured
-        // from the pools' streams into ONE hipGraph and launched once
-        // (TPT_BATCH_GRAPH=0: the per-pool wave graphs are replayed from the host
-        // instead).  Completion times come from device clock stamps: [0] the
-        // start, [1 + w] after batched wave w, [1 + W + i] after large scene i.
+        // The render: the batched scenes' pipeline waves (one captured wave per
+        // pool, replayed from the host), then the large scenes' plain waves, over
+        // the pools.  TPT_BATCH_GRAPH=1 captures the whole render from the pools'
+        // streams into ONE hipGraph and launches it once instead: the images are
+        // the same, but ROCm's graph executor overlaps the pools' kernels less
+        // than the streams do (64 mixed rooms 3,141 against 3,107 ms; the city at
+        // 256 spp as a batch of one 1,413 against 1,055 ms).  Completion times
+        // come from device clock stamps: [0] the start, [1 + w] after batched
+        // wave w, [1 + W + i] after large scene i.
         auto tr = std::chrono::steady_clock::now();
-        const bool one_graph = env_int("TPT_BATCH_GRAPH", 1, 0, 1) != 0;
+        const bool one_graph = env_int("TPT_BATCH_GRAPH", 0, 0, 1) != 0;
         int plain_waves = 0;
         hipGraphExec_t ge[BatchRun::kMaxPools] = {};
         const int W = nb > 0 ? run.waves(best_cap) : 0;
