@@ -13,9 +13,9 @@
 // MFMA 16x16x32 lane layout (wave64, lane l, j = 0..7 / 0..3):
 //   A[m = l % 16][k = (l / 16) * 8 + j]     B[k = (l / 16) * 8 + j][n = l % 16]
 //   D[m = (l / 16) * 4 + j][n = l % 16]
-// A wavefront processes a tile of 64 samples: M = 4 tiles of 16 samples,
-// N = 4 tiles of 16 features, K = 2 steps of 32 features.  Activations of the
-// tile are staged in LDS as [sample][feature] BF16 rows (stride kRow).
+// Inference keeps a tile's activations in registers (k_nrc_infer); training
+// stages a 64-sample tile's activations in LDS as [sample][feature] BF16 rows
+// (stride kRow) and splits its 16 x 16 output blocks over four wavefronts.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -51,60 +51,12 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// One forward layer for the wavefront's 64-sample tile: out_lds = relu(in_lds . W)
-// (hidden layers) or, for the last layer, the three outputs in `pred`.
-template <bool LAST>
-__device__ __forceinline__ void layer_fwd(const uint16_t *in_lds, uint16_t *out_lds, const uint16_t *wt,
-                                          float pred[4][4]) {
-    const int l = threadIdx.x & 63;
-    const int r = l & 15, q = l >> 4;
-    constexpr int NT = LAST ? 1 : 4;
-    f32x4 acc[4][NT];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < 2; ks++) {
-        bf16x8 b[NT];
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) b[nt] = ld8(wt + (nt * 16 + r) * 64 + ks * 32 + q * 8);
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            bf16x8 a = ld8_lds(in_lds + (mt * 16 + r) * kRow + ks * 32 + q * 8);
-#pragma unroll
-            for (int nt = 0; nt < NT; nt++) acc[mt][nt] = mfma(a, b[nt], acc[mt][nt]);
-        }
-    }
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float v = acc[mt][nt][j];
-                if (LAST) pred[mt][j] = v; // sample mt*16 + q*4 + j, output n = r (valid for r < 3)
-                else out_lds[(mt * 16 + q * 4 + j) * kRow + nt * 16 + r] = f2bf(fmaxf(v, 0.f));
-            }
-}
-
 // ---------------------------------------------------------------- inference
 // Record count (clamped to n_max) and the feature row of record i.
 __device__ __forceinline__ int rec_count(const uint32_t *n_dev, int n_max) { return min((int)*n_dev, n_max); }
 __device__ __forceinline__ const uint16_t *rec_row(const uint16_t *feats, const int32_t *idx, int i) {
     return feats + (size_t)(idx ? idx[i] : i) * 64;
 }
-// Loads the feature rows of records [tile0, tile0 + 64) into an LDS tile (zeros beyond n).
-__device__ __forceinline__ void load_tile(uint16_t *dst, const uint16_t *feats, const int32_t *idx, int tile0, int n) {
-    const int l = threadIdx.x & 63;
-    for (int i = l; i < 64 * 8; i += 64) {
-        int s = i >> 3, c = (i & 7) * 8;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (tile0 + s < n) v = *reinterpret_cast<const uint4 *>(rec_row(feats, idx, tile0 + s) + c);
-        *reinterpret_cast<uint4 *>(dst + s * kRow + c) = v;
-    }
-}
-
 // out[i]: predicted radiance (rgb, 0) of record i.
 //
 // The activations stay in registers: each layer computes y^T = W^T x^T with A =
