@@ -102,6 +102,28 @@ def test_mlp_inference_matches_reference(hip, dev):
         assert rel.max() < 1e-2
 
 
+def test_mlp_inference_persistent_tiles(hip, dev):
+    # more records than one pass of the persistent inference wavefronts takes
+    # (2048 wavefronts x 32 records per CU-filling launch): each wavefront walks
+    # several tiles, loading the next tile's rows while one computes; a sample of
+    # the outputs, spread over the whole range, against the reference
+    rng = np.random.default_rng(4)
+    W = he_weights(rng)
+    hip.cache_weights(dev, set=W)
+    n = 300_001  # a partial last tile
+    x = bf16(rng.uniform(-1, 1, (n, 64)).astype(np.float32))
+    got = hip.cache_eval(dev, bits(x))
+    pick = np.unique(np.concatenate([rng.integers(0, n, 1500), np.arange(n - 40, n), np.arange(0, 40)]))
+    ref, _ = forward(W, x[pick])
+    scale = np.abs(ref).max() + 1e-6
+    rel = np.abs(got[pick] - ref) / scale
+    within = float(np.mean(rel <= 1e-5))
+    print(f"n={n}: {len(pick)} sampled outputs, max rel {rel.max():.2e}, within 1e-5: {within:.4f}")
+    assert np.all(np.isfinite(got))
+    assert within >= 0.99
+    assert rel.max() < 1e-2
+
+
 def test_mlp_gradient_matches_reference(hip, dev):
     rng = np.random.default_rng(2)
     W = he_weights(rng)
