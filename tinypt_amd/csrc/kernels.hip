@@ -1058,13 +1058,18 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
                 nrc_encode((record ? C->feat_t : C->feat) + (size_t)slot * kNrcInputs, p01, neg(d), sh.shade_normal,
                            rough, alb);
             }
+            // with the albedo factor, the throughput into the vertex includes the albedo:
+            // the recorded target becomes radiance / albedo and a query's prediction
+            // is multiplied by it
+            const V3 tf = C->albedo_factor ? cmul(decay, alb) : decay;
             if (record) {
                 out_.cache_kind = 2;
-                C->rec_T[slot] = make_float4(decay.x, decay.y, decay.z, (float)depth);
+                C->rec_T[slot] = make_float4(tf.x, tf.y, tf.z, (float)depth);
                 C->rec_C[slot] = make_float4(color.x, color.y, color.z, 0.f);
             } else if (query) {
                 out_.cache_kind = 1;
-                into_cache = true; // P.decay[slot] keeps the throughput arriving here
+                into_cache = true; // P.decay[slot]: the throughput the prediction is applied with
+                if (C->albedo_factor) P.decay[slot] = make_float4(tf.x, tf.y, tf.z, 0.f);
             } else if (trains && depth == C->depth) {
                 C->rec_T[slot] = make_float4(0.f, 0.f, 0.f, -1.f); // nothing recorded yet
             }
