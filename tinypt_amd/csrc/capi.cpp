@@ -1540,12 +1540,14 @@ void setup_frames(tpt_hip_scene *h, const tpt_render_params *p, const tpt_cache_
         // 0.0161 to 0.0143 but take 33% longer per frame (paths go on through mirrors
         // and glass), so the default 0 ends every path at the cache depth
         C.min_rough = (float)env_int("TPT_CACHE_MIN_ROUGH", 0, 0, 100) / 100.f;
-        // the network learns radiance / albedo at the cache vertex (TPT_NRC_ALBEDO=0: radiance).
-        // Textured room 1920x1080, 4 runs against a 16k-spp render: relMSE at 256 frames 3.4-5.1%
-        // lower, at 16 / 64 frames 1-2.6% higher.  A moving average of the weights for the
-        // queries (decay 0.95 / 0.99) lagged the still-improving network (-19% / -25% bias at 16
-        // frames) and did not lower the 256-frame error, so queries read the latest weights.
-        C.albedo_factor = env_int("TPT_NRC_ALBEDO", 1, 0, 1);
+        // where the cache vertex's first lobe takes its colour from a texture, the network
+        // learns radiance / albedo (TPT_NRC_ALBEDO: 0 nowhere, 1 textured lobes, 2 every
+        // vertex).  Textured room 1920x1080, 4 runs each against a 16k-spp render: relMSE at
+        // 256 frames 0.01796 / 0.01665 / 0.01732 (mean), at 64 frames 0.0435 / 0.0450 /
+        // 0.0452; untextured scenes are unchanged by 1.  A moving average of the weights for
+        // the queries (decay 0.95 / 0.99) lagged the still-improving network (-19% / -25%
+        // bias at 16 frames) and did not lower the 256-frame error.
+        C.albedo_factor = env_int("TPT_NRC_ALBEDO", 1, 0, 2);
     }
     // With the cache ending every path at its cache vertex (depth cache_depth, any
     // material) and training paths suffix_len vertices later, no path is alive past

@@ -248,8 +248,9 @@ struct DCache {
     int32_t suffix;       // > 0: training paths terminate into the cache `suffix` vertices after their cache vertex
     float min_rough;      // a path ends in the cache at its first vertex of depth >= `depth` whose first
                           // lobe's roughness is >= min_rough (0: at depth `depth`, whatever the material)
-    int32_t albedo_factor; // 1: the network predicts the vertex's scattered radiance divided by its first
-                           // lobe's albedo (queries multiply it back): textures need not be learned
+    int32_t albedo_factor; // the network predicts the vertex's scattered radiance divided by its first
+                           // lobe's albedo (queries multiply it back), so textures need not be learned:
+                           // 1 where that lobe's colour is a texture map, 2 at every vertex, 0 nowhere
     uint16_t *feat;       // per slot: 64 BF16 inputs of the vertex where the path queries the cache
     uint16_t *feat_t;     // per slot: 64 BF16 inputs of a training path's cache vertex
     float4 *rec_T;        // per slot: throughput arriving at the vertex, w = its depth (-1: none yet)

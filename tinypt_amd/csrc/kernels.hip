@@ -1061,7 +1061,8 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
             // with the albedo factor, the throughput into the vertex includes the albedo:
             // the recorded target becomes radiance / albedo and a query's prediction
             // is multiplied by it
-            const V3 tf = C->albedo_factor ? cmul(decay, alb) : decay;
+            const bool factor = C->albedo_factor == 2 || (C->albedo_factor == 1 && m.lobe[0].color.image >= 0);
+            const V3 tf = factor ? cmul(decay, alb) : decay;
             if (record) {
                 out_.cache_kind = 2;
                 C->rec_T[slot] = make_float4(tf.x, tf.y, tf.z, (float)depth);
@@ -1069,7 +1070,7 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
             } else if (query) {
                 out_.cache_kind = 1;
                 into_cache = true; // P.decay[slot]: the throughput the prediction is applied with
-                if (C->albedo_factor) P.decay[slot] = make_float4(tf.x, tf.y, tf.z, 0.f);
+                if (factor) P.decay[slot] = make_float4(tf.x, tf.y, tf.z, 0.f);
             } else if (trains && depth == C->depth) {
                 C->rec_T[slot] = make_float4(0.f, 0.f, 0.f, -1.f); // nothing recorded yet
             }
