@@ -121,6 +121,16 @@ def cpu_baseline(spp=64, rows=256, width=1024, height=768):
     oracle.render(height, p, y0, y0 + rows, threads=threads)
     dt = time.perf_counter() - t0
     paths = width * rows * spp
+    # the extend baseline: the oracle's closest hits (the reference structure: linear top
+    # level, median-split BVHTree per shape) over the 1M-ray set of the parity tests
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import raysets
+
+    cam = host.camera()
+    rays = raysets.extend_set(oracle, (cam.location, cam.rotation, width, height, cam.fov), 1_000_000, seed=0x5EED)
+    t0 = time.perf_counter()
+    oracle.trace_closest(rays, mode=0, threads=threads)
+    dt_rays = time.perf_counter() - t0
     return {
         "value": paths / dt / 1e6,
         "unit": "Mpaths/s",
@@ -129,6 +139,8 @@ def cpu_baseline(spp=64, rows=256, width=1024, height=768):
         "sample": f"triangle room rows {y0}-{y0 + rows} of 1024x768 at {spp} spp, depth 8 ({paths} paths, {dt:.1f} s)",
         "cpu": model,
         "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp",
+        "extend": {"value": round(len(rays) / dt_rays / 1e6, 2), "unit": "Mrays/s",
+                   "sample": f"closest hits of the 1M-ray set (500k camera + 500k cosine secondary rays), {dt_rays:.2f} s"},
     }
 
 
