@@ -606,7 +606,7 @@ def bench_city(args):
     """The ~6M-triangle city, 1920x1080 at 1024 spp, depth 8: the plain tracer at the
     configuration's 1024 spp (timed: `value`), the batched pipeline (one captured
     pipeline wave per pool, replayed) on the same render, and the learned radiance
-    cache, each judged by relMSE against a plain FP64 render of --ref-spp samples
+    cache, each measured by relMSE against a plain FP64 render of --ref-spp samples
     per pixel (16,384 by default)."""
     import numpy as np
 
