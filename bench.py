@@ -56,11 +56,11 @@ RAY_BYTES, TRI_BYTES = 52, 48
 # the workload they were taken on.
 # Room 1024x768, 256 spp, depth 8: `bench.py --steps 1 --warmup 0 --no-cpu --no-count --no-cache`
 # (117 dispatches).
-EXTEND_COUNTERS = {"valu_insts": 1085295525, "thread_cycles": 37099849767, "active_inst": 1090375650,
-                   "fetch_kib": 1637813.2, "write_kib": 306847.6}
+EXTEND_COUNTERS = {"valu_insts": 1122033576, "thread_cycles": 37290147763, "active_inst": 1127273312,
+                   "fetch_kib": 1632454.4, "write_kib": 306188.9}
 # City 1920x1080, depth 8: one 16-spp render (18 dispatches, waves of 8 samples per pixel).
-CITY_EXTEND_COUNTERS = {"valu_insts": 1373123544, "thread_cycles": 49709677644, "active_inst": 1377392649,
-                        "fetch_kib": 3081072.1, "write_kib": 263549.7}
+CITY_EXTEND_COUNTERS = {"valu_insts": 1413430719, "thread_cycles": 49919942694, "active_inst": 1417850773,
+                        "fetch_kib": 3085105.1, "write_kib": 262906.4}
 # Two-spheres box 512x384, 64 spp, depth 10, RR 5: the shade kernel's counters summed over the
 # 11 dispatches of one render (shade is 88% of the render's kernel time: the spheres' hits are
 # computed where rays are made).

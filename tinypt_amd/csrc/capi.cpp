@@ -250,11 +250,12 @@ void upload_scene(tpt_hip_scene *h, const HostScene &H) {
     // the queue and serialises every refill at one L2 channel (L2 read latency 700 vs 233 cycles).
     S.persistent = 4;
     if (pers && std::string(pers).size() == 1 && pers[0] >= '0' && pers[0] <= '4') S.persistent = pers[0] - '0';
-    // room 1024x768x256, three streams: chunks of 384 / 768 / 1024 give 703-707 /
-    // 707-720 / 713 Mpaths/s, and the co-scheduled extend launches run at 0.66 /
-    // 0.70-0.75 / 0.71 of the roofline (larger chunks leave more CUs to the other
-    // streams' kernels)
-    S.chunk = env_int("TPT_CHUNK", 768, 64, 1 << 20) / 64 * 64;
+    // queue entries per wavefront of the chunked traversal.  Room 1024x768x256, depth 8:
+    // chunks of 256 / 384 / 512 / 640 / 768 / 1024 / 1536 give three-stream frames of
+    // 620-622 / 628-630 / 625-630 / 624-629 / 619-625 / 615 / 595 Mpaths/s and extend alone
+    // 229 / 227 / 228 / 233 / 239 / 247 / 257 ms per frame; city 1920x1080x64: 512 and 768
+    // give the same frame, extend alone 186 against 195 ms
+    S.chunk = env_int("TPT_CHUNK", 512, 64, 1 << 20) / 64 * 64;
     // traversal stack: the first kLdsStack entries in LDS by default (the private
     // scratch stack spills to memory: ~9x the extend kernel's HBM writes);
     // TPT_LDS_STACK=0 selects the private stack
