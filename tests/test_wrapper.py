@@ -73,3 +73,22 @@ def test_extras_build_and_render_in_the_oracle():
     s = W.cornell_wrapper(32, 24, extras=True)
     img = O.OracleScene(s._obj).render(24, O.params(2, seed=1, max_depth=4, hdr=True))
     assert np.all(np.isfinite(img)) and img.mean() > 0.0
+
+
+def test_mesh_from_obj_through_the_wrapper(tmp_path):
+    # TriangleMesh.from_obj through the wrapper (wrapper.py:200-205) loads the file as _C does
+    from tinypt_amd import _C
+
+    (tmp_path / "quad.mtl").write_text("newmtl red\nKd 0.8 0.1 0.1\n")
+    (tmp_path / "quad.obj").write_text("mtllib quad.mtl\nv 0 0 0\nv 1 0 0\nv 1 0 1\nv 0 0 1\nusemtl red\nf 1 2 3 4\n")
+    path = str(tmp_path / "quad.obj")
+    mesh = tinypt.TriangleMesh.from_obj(path)
+    assert mesh._obj.num_faces == 2  # the quad split in two
+    cam = tinypt.Camera(location=(0.5, 3, 0.5), rotation=W.rot_x(90) @ W.rot_y(180), resolution=(8, 6), fov=0.8)
+    light = dict(radius=0.4, location=(0.5, 1.5, 1.2), emission=(5, 5, 5))
+    a = tinypt.Scene(camera=cam, objects=[mesh, tinypt.Sphere(**light)])
+    b = _C.Scene(cam._obj, [_C.TriangleMesh.from_obj(path, None), tinypt.Sphere(**light)._obj], [],
+                 _C.EnvTexture(_C.RGBTexture([0.0, 0.0, 0.0])))
+    p = O.params(4, seed=2, max_depth=3, hdr=True)
+    ia, ib = O.OracleScene(a._obj).render(6, p), O.OracleScene(b).render(6, p)
+    assert ia.max() > 0 and np.array_equal(ia, ib)
