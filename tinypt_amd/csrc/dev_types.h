@@ -251,6 +251,7 @@ struct DCache {
     int32_t albedo_factor; // the network predicts the vertex's scattered radiance divided by its first
                            // lobe's albedo (queries multiply it back), so textures need not be learned:
                            // 1 where that lobe's colour is a texture map, 2 at every vertex, 0 nowhere
+    float light_prob;      // probability of sampling the light mixture at a vertex before the cache
     uint16_t *feat;       // per slot: 64 BF16 inputs of the vertex where the path queries the cache
     uint16_t *feat_t;     // per slot: 64 BF16 inputs of a training path's cache vertex
     float4 *rec_T;        // per slot: throughput arriving at the vertex, w = its depth (-1: none yet)

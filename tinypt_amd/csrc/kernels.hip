@@ -1019,7 +1019,8 @@ __device__ __forceinline__ bool cache_trains(const DCache &C, int slot) {
 
 __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &R, const DPaths &P, uint32_t i,
                                                 int slot, const DCache *C = nullptr) {
-    const float LIGHT_PROB = 1.f / 5.f; // cpu/path_tracer.cpp:104
+    // cpu/path_tracer.cpp:104; cached frames may sample the lights more often
+    const float LIGHT_PROB = C ? C->light_prob : 1.f / 5.f;
     ShadeOut out_{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
     bool &alive = out_.alive, &emit_shadow = out_.emit_shadow;
     V3 &s_org = out_.s_org, &s_dir = out_.s_dir, &s_val = out_.s_val;
