@@ -165,6 +165,28 @@ def test_traversal_kernels_render_identically(hip, monkeypatch):
         assert np.array_equal(images[0], img)
 
 
+@pytest.mark.parametrize("case", ["spheres_rr_hdr", "spheres_b3_ldr", "wrapper_extras_fp64"])
+def test_whole_path_kernel_matches_wavefront(hip, monkeypatch, case):
+    # scenes without a BVH trace their paths whole (k_megapath); the wavefront
+    # kernels (TPT_MEGAKERNEL=0) give the same image bit for bit
+    import wrapper_scenes as W
+    from tinypt_amd import scenes
+
+    if case == "spheres_rr_hdr":
+        host, spp, kw = scenes.cornell_spheres(96, 72), 16, dict(max_depth=10, rr_depth=5, hdr=True)
+    elif case == "spheres_b3_ldr":  # 150 spp: 3 x 3 sub-pixels of 16 samples
+        host, spp, kw = scenes.cornell_spheres(40, 30), 150, dict(max_depth=6)
+    else:
+        host, spp, kw = W.cornell_wrapper(64, 48, extras=True)._obj, 8, dict(max_depth=8, hdr=True, fp64=True)
+    dev = host.to("hip")
+    images = []
+    for mode in "10":
+        monkeypatch.setenv("TPT_MEGAKERNEL", mode)
+        images.append(hip.PathTracer("hip").render(dev, spp, seed=6, **kw).raw())
+    assert images[0].mean() > 0 and np.all(np.isfinite(images[0]))
+    assert np.array_equal(images[0], images[1])
+
+
 def test_obj_scene_matches_oracle(hip, tmp_path):
     # a room loaded through TriangleMesh.from_obj (y-up OBJ, quads, an MTL with a
     # textured floor, a glossy wall, metal and glass boxes) renders like the oracle

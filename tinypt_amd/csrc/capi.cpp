@@ -25,6 +25,7 @@
 
 namespace tpt {
 void launch_generate(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, hipStream_t);
+void launch_megapath(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, int, hipStream_t);
 void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, uint64_t, hipStream_t);
 void launch_shade(const DScene &, const DRender &, const DPaths &, const int32_t *, int32_t *, int, int, int, int,
                   hipStream_t, const DCache *);
@@ -546,11 +547,15 @@ int grid_for(size_t n, int block, int cus = 256) {
 // accumulate waits for the previous one's (`prev_acc`: index into `evs` of that
 // accumulate's event, -1 before the first wave; updated) and the samples are added
 // in order.  With `timing`, events bracket every extend / shade / shadow launch.
+// A scene without a BVH (analytic primitives only) traces each wave's paths whole
+// in k_megapath instead (same colours; `timing` brackets it as shading), unless
+// traffic is counted or TPT_MEGAKERNEL=0.
 struct PlainWaves {
     DPaths pools[kMaxLanes] = {};
     hipStream_t streams[kMaxLanes] = {};
     int npools = 1, block = 256, cus = 256;
     bool count = false, timing = false;
+    bool megakernel = env_int("TPT_MEGAKERNEL", 1, 0, 1) != 0;
     std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
     int waves = 0;
     static int record(EventVec &evs, hipStream_t s, bool timing) {
@@ -571,25 +576,31 @@ struct PlainWaves {
             auto ev = [&]() { return record(evs, sw, timing); };
             int k_active = std::min(k, R.spp_eff - wave_base);
             int n = pixels * k_active;
-            launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);
             int grid = grid_for((size_t)n, block, cus);
-            int cur = 0;
-            for (int b = 0; b <= R.max_depth; b++) {
-                int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
-                int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+            if (megakernel && S.root == -1 && !count) {
                 int e0 = timing ? ev() : -1;
-                launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, sw);
-                int e1 = timing ? ev() : -1;
-                launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, sw, nullptr);
-                int e2 = timing ? ev() : -1;
-                launch_shadow(S, P, grid, block, count, (uint64_t)n, sw);
-                int e3 = timing ? ev() : -1;
-                if (timing) {
-                    ext_ev.push_back({e0, e1});
-                    shade_ev.push_back({e1, e2});
-                    shadow_ev.push_back({e2, e3});
+                launch_megapath(S, R, P, wave_base, n, base_mixed, grid, block, sw);
+                if (timing) shade_ev.push_back({e0, ev()});
+            } else {
+                launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);
+                int cur = 0;
+                for (int b = 0; b <= R.max_depth; b++) {
+                    int32_t *q = cur == 0 ? P.queue_a : P.queue_b;
+                    int32_t *nx = cur == 0 ? P.queue_b : P.queue_a;
+                    int e0 = timing ? ev() : -1;
+                    launch_extend(S, P, q, cur, 1 - cur, grid, block, count, (uint64_t)n, sw);
+                    int e1 = timing ? ev() : -1;
+                    launch_shade(S, R, P, q, nx, cur, 1 - cur, grid, block, sw, nullptr);
+                    int e2 = timing ? ev() : -1;
+                    launch_shadow(S, P, grid, block, count, (uint64_t)n, sw);
+                    int e3 = timing ? ev() : -1;
+                    if (timing) {
+                        ext_ev.push_back({e0, e1});
+                        shade_ev.push_back({e1, e2});
+                        shadow_ev.push_back({e2, e3});
+                    }
+                    cur = 1 - cur;
                 }
-                cur = 1 - cur;
             }
             if (npools > 1 && prev_acc >= 0) HIP_OK(hipStreamWaitEvent(sw, evs[prev_acc], 0));
             launch_accumulate(R, P, wave_base, k_active, w == 0 ? 1 : 0, sw);

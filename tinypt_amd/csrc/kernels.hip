@@ -849,12 +849,14 @@ __device__ __forceinline__ void put_ray(const DScene &S, const DPaths &P, int sl
     P.dir[slot] = make_float4(d.x, d.y, d.z, __int_as_float(code));
 }
 
-__device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, const DPaths &P, int j, int pix, int s,
-                                         uint64_t base_mixed) {
+// Camera ray of sample s of pixel pix with the path's stream seed `key` and its
+// generator after the ray's draws.
+__device__ __forceinline__ void camera_ray(const DScene &S, const DRender &R, int pix, int s, uint64_t base_mixed,
+                                           V3 &org, V3 &dir, uint64_t &key, Rng &rng) {
     int W = S.cam.width;
     int x = pix % W, y = pix / W;
-    uint64_t key = stream_seed(base_mixed, (uint64_t)pix, (uint64_t)s);
-    Rng rng{rng_state_from_seed(key)};
+    key = stream_seed(base_mixed, (uint64_t)pix, (uint64_t)s);
+    rng = Rng{rng_state_from_seed(key)};
     // PixelSampler sub-pixel origin (path_tracer.cpp:8-58): repeated additions of 1/b
     int subpix = s / R.nsub;
     int sx = subpix % R.block, sy = subpix / R.block;
@@ -872,9 +874,17 @@ __device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, cons
         offset = add(scl(uo.x, mk(c.u)), scl(uo.y, mk(c.v)));
     }
     float a = px / (float)c.width - .5f, b = py / (float)c.height - .5f;
-    V3 dir = sub(add(add(scl(a, mk(c.right)), scl(b, mk(c.up))), mk(c.direct)), offset);
+    dir = sub(add(add(scl(a, mk(c.right)), scl(b, mk(c.up))), mk(c.direct)), offset);
     dir = nrm(dir);
-    V3 org = add(mk(c.center), offset);
+    org = add(mk(c.center), offset);
+}
+
+__device__ __forceinline__ void gen_path(const DScene &S, const DRender &R, const DPaths &P, int j, int pix, int s,
+                                         uint64_t base_mixed) {
+    V3 org, dir;
+    uint64_t key;
+    Rng rng;
+    camera_ray(S, R, pix, s, base_mixed, org, dir, key, rng);
     put_ray(S, P, j, org, dir, key, 0);
     P.decay[j] = make_float4(1.f, 1.f, 1.f, 0.f);
     P.color[j] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1017,21 +1027,24 @@ __device__ __forceinline__ bool cache_trains(const DCache &C, int slot) {
     return splitmix64(((uint64_t)C.frame << 32) ^ (uint64_t)(uint32_t)slot) % (uint64_t)C.train_stride == 0;
 }
 
-__device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &R, const DPaths &P, uint32_t i,
-                                                int slot, const DCache *C = nullptr) {
+// The state of the path being shaded, read and written through `Path`: PoolPath
+// (the wavefront kernels: the path pool, the hit record at queue position i) or
+// RegPath (k_megapath: registers).  Returns the shadow ray to trace, if any.
+template <class Path>
+__device__ __forceinline__ ShadeOut shade_path(const DScene &S, const DRender &R, Path &st, const DCache *C) {
+    const int slot = st.slot;
     // cpu/path_tracer.cpp:104; cached frames may sample the lights more often
     const float LIGHT_PROB = C ? C->light_prob : 1.f / 5.f;
     ShadeOut out_{false, false, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, kINF, 0};
     bool &alive = out_.alive, &emit_shadow = out_.emit_shadow;
     V3 &s_org = out_.s_org, &s_dir = out_.s_dir, &s_val = out_.s_val;
     float &s_tmax = out_.s_tmax;
-    float4 h4 = P.hit[i];
-    HitRec h{h4.x, -1, h4.z, h4.w, __float_as_int(h4.y)}; // the path pipeline's hit records carry the local id
-    V3 o = mk4(P.org[slot]), d = mk4(P.dir[slot]);
-    V3 decay = mk4(P.decay[slot]);
-    V3 color = mk4(P.color[slot]);
-    int depth = P.depth[slot];
-    Rng rng{P.rng[slot]};
+    HitRec h = st.hit();
+    V3 o = st.org(), d = st.dir();
+    V3 decay = st.decay();
+    V3 color = st.color();
+    int depth = st.depth();
+    Rng rng{st.rng()};
     if (!(h.t < kINF)) {
         color = add(color, cmul(decay, background(S, d)));
     } else {
@@ -1074,7 +1087,7 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
             } else if (query) {
                 out_.cache_kind = 1;
                 into_cache = true; // P.decay[slot]: the throughput the prediction is applied with
-                if (factor) P.decay[slot] = make_float4(tf.x, tf.y, tf.z, 0.f);
+                if (factor) st.set_decay(tf);
             } else if (trains && depth == C->depth) {
                 C->rec_T[slot] = make_float4(0.f, 0.f, 0.f, -1.f); // nothing recorded yet
             }
@@ -1132,15 +1145,45 @@ __device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &
                 }
             }
             if (alive) {
-                put_ray(S, P, slot, hp, out, P.key[slot], depth + 1);
-                P.decay[slot] = make_float4(decay.x, decay.y, decay.z, 0.f);
-                P.depth[slot] = depth + 1;
+                st.next_ray(S, hp, out, depth + 1);
+                st.set_decay(decay);
+                st.set_depth(depth + 1);
             }
         }
     }
-    P.color[slot] = make_float4(color.x, color.y, color.z, 0.f);
-    P.rng[slot] = rng.x;
+    st.set_color(color);
+    st.set_rng(rng.x);
     return out_;
+}
+
+// Path state in the path pool; the hit record of the path at queue position i.
+struct PoolPath {
+    const DPaths &P;
+    uint32_t i;
+    int slot;
+    __device__ __forceinline__ HitRec hit() const {
+        float4 h4 = P.hit[i];
+        return HitRec{h4.x, -1, h4.z, h4.w, __float_as_int(h4.y)}; // the path pipeline's hit records carry the local id
+    }
+    __device__ __forceinline__ V3 org() const { return mk4(P.org[slot]); }
+    __device__ __forceinline__ V3 dir() const { return mk4(P.dir[slot]); }
+    __device__ __forceinline__ V3 decay() const { return mk4(P.decay[slot]); }
+    __device__ __forceinline__ V3 color() const { return mk4(P.color[slot]); }
+    __device__ __forceinline__ int depth() const { return P.depth[slot]; }
+    __device__ __forceinline__ uint64_t rng() const { return P.rng[slot]; }
+    __device__ __forceinline__ void next_ray(const DScene &S, V3 o, V3 d, int depth) const {
+        put_ray(S, P, slot, o, d, P.key[slot], depth);
+    }
+    __device__ __forceinline__ void set_decay(V3 v) const { P.decay[slot] = make_float4(v.x, v.y, v.z, 0.f); }
+    __device__ __forceinline__ void set_depth(int v) const { P.depth[slot] = v; }
+    __device__ __forceinline__ void set_color(V3 c) const { P.color[slot] = make_float4(c.x, c.y, c.z, 0.f); }
+    __device__ __forceinline__ void set_rng(uint64_t x) const { P.rng[slot] = x; }
+};
+
+__device__ __forceinline__ ShadeOut shade_entry(const DScene &S, const DRender &R, const DPaths &P, uint32_t i,
+                                                int slot, const DCache *C = nullptr) {
+    PoolPath pp{P, i, slot};
+    return shade_path(S, R, pp, C);
 }
 
 // Extra attributes of the shade kernels (occupancy-tuning builds define it, e.g.
@@ -1184,6 +1227,68 @@ __global__ TPT_SHADE_ATTR void k_shade(DScene S, DRender R, DPaths P, const int3
             P.shadow_dir[spos] = make_float4(r.s_dir.x, r.s_dir.y, r.s_dir.z, 0.f);
             P.shadow_val[spos] = make_float4(r.s_val.x, r.s_val.y, r.s_val.z, 0.f);
         }
+    }
+}
+
+// ---------------------------------------------------------------- whole paths
+// Path state in registers (k_megapath).
+struct RegPath {
+    int slot;
+    HitRec h;
+    V3 o, d, dec, col;
+    int dep;
+    uint64_t rs, key;
+    __device__ __forceinline__ HitRec hit() const { return h; }
+    __device__ __forceinline__ V3 org() const { return o; }
+    __device__ __forceinline__ V3 dir() const { return d; }
+    __device__ __forceinline__ V3 decay() const { return dec; }
+    __device__ __forceinline__ V3 color() const { return col; }
+    __device__ __forceinline__ int depth() const { return dep; }
+    __device__ __forceinline__ uint64_t rng() const { return rs; }
+    __device__ __forceinline__ void next_ray(const DScene &, V3 no, V3 nd, int) {
+        o = no;
+        d = nd;
+    }
+    __device__ __forceinline__ void set_decay(V3 v) { dec = v; }
+    __device__ __forceinline__ void set_depth(int v) { dep = v; }
+    __device__ __forceinline__ void set_color(V3 c) { col = c; }
+    __device__ __forceinline__ void set_rng(uint64_t x) { rs = x; }
+};
+
+// Scenes of analytic primitives only (no BVH): each thread traces path j of the
+// wave from its camera ray to its end with the state in registers (the shape of
+// the reference's CUDA kernel, cuda/path_tracer_kernel.cu:80), instead of passing it
+// through the pool between the wavefront kernels at every bounce.  A ray's
+// closest hit is the analytic list's (put_ray's test, same alpha context), a
+// shadow ray's occlusion the list's any-hit test (the shadow kernels' ray_start),
+// and the radiance is added in the wavefront's order (the vertex's emission and
+// BxDF sample in shade_path, then its shadow ray's contribution), so each path's
+// colour equals the wavefront kernels' bit for bit.  Writes it to P.color[j] for
+// k_accumulate.
+__global__ TPT_SHADE_ATTR void k_megapath(DScene S, DRender R, DPaths P, int wave_base, int n_paths,
+                                          uint64_t base_mixed) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_paths; j += gridDim.x * blockDim.x) {
+        RegPath st;
+        Rng rng;
+        camera_ray(S, R, j % R.pixels, wave_base + j / R.pixels, base_mixed, st.o, st.d, st.key, rng);
+        st.slot = j;
+        st.rs = rng.x;
+        st.dec = V3{1.f, 1.f, 1.f};
+        st.col = V3{0.f, 0.f, 0.f};
+        st.dep = 0;
+        while (true) {
+            HitRec a{kINF, -1, 0.f, 0.f, 0};
+            analytic_tests<false>(S, st.o, st.d, AlphaCtx{st.key, st.dep, true}, a);
+            st.h = HitRec{a.t, -1, a.u, a.v, a.loc};
+            const ShadeOut r = shade_path(S, R, st, nullptr);
+            if (r.emit_shadow) {
+                HitRec b{r.s_tmax, -1, 0.f, 0.f, 0};
+                if (!analytic_tests<true>(S, r.s_org, r.s_dir, AlphaCtx{0, 0, false}, b))
+                    st.col = V3{st.col.x + r.s_val.x, st.col.y + r.s_val.y, st.col.z + r.s_val.z};
+            }
+            if (!r.alive) break;
+        }
+        P.color[j] = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
     }
 }
 
@@ -1781,6 +1886,10 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
     int grid = (n_paths + block - 1) / block;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+}
+void launch_megapath(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
+                     uint64_t base_mixed, int grid, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_megapath, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
 }
 // Compile-time dispatch on the node layout and on boolean kernel variants.
 template <int V> using IntC = std::integral_constant<int, V>;
