@@ -25,7 +25,7 @@
 
 namespace tpt {
 void launch_generate(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, hipStream_t);
-void launch_megapath(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, int, hipStream_t);
+void launch_megapath(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, int, bool, hipStream_t);
 void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, uint64_t, hipStream_t);
 void launch_shade(const DScene &, const DRender &, const DPaths &, const int32_t *, int32_t *, int, int, int, int,
                   hipStream_t, const DCache *);
@@ -556,6 +556,9 @@ struct PlainWaves {
     int npools = 1, block = 256, cus = 256;
     bool count = false, timing = false;
     bool megakernel = env_int("TPT_MEGAKERNEL", 1, 0, 1) != 0;
+    // k_megapath with lane refill: waves per CU of its grid (0, the default and the
+    // fastest measured: one thread per path, no refill)
+    int mega_waves = env_int("TPT_MEGA_WAVES", 0, 0, 1024);
     std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
     int waves = 0;
     static int record(EventVec &evs, hipStream_t s, bool timing) {
@@ -579,7 +582,9 @@ struct PlainWaves {
             int grid = grid_for((size_t)n, block, cus);
             if (megakernel && S.root == -1 && !count) {
                 int e0 = timing ? ev() : -1;
-                launch_megapath(S, R, P, wave_base, n, base_mixed, grid, block, sw);
+                const bool refill = mega_waves > 0;
+                const int mgrid = refill ? std::max(1, std::min(grid, cus * mega_waves * 64 / block)) : grid;
+                launch_megapath(S, R, P, wave_base, n, base_mixed, mgrid, block, refill, sw);
                 if (timing) shade_ev.push_back({e0, ev()});
             } else {
                 launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);

@@ -1264,31 +1264,61 @@ struct RegPath {
 // and the radiance is added in the wavefront's order (the vertex's emission and
 // BxDF sample in shade_path, then its shadow ray's contribution), so each path's
 // colour equals the wavefront kernels' bit for bit.  Writes it to P.color[j] for
-// k_accumulate.
+// k_accumulate.  REFILL: a lane whose path ends starts its next path (j + the
+// grid's thread count) at the next bounce, instead of waiting for the wave's
+// longest path; the grid is then sized to the resident waves.  Measured slower
+// on the two-spheres box 512x384x64 (depth 10, RR 5; same images): one thread
+// per path 4.66 ms per render, refill with 8 / 16 / 32 / 64 waves per CU 8.3 /
+// 5.5 / 5.2 / 4.95 ms.  New waves starting where others end keep the SIMDs as
+// busy as refilled lanes do, without the divergent path starts.
+__device__ __forceinline__ void mega_start(const DScene &S, const DRender &R, int j, int wave_base,
+                                           uint64_t base_mixed, RegPath &st) {
+    Rng rng;
+    camera_ray(S, R, j % R.pixels, wave_base + j / R.pixels, base_mixed, st.o, st.d, st.key, rng);
+    st.slot = j;
+    st.rs = rng.x;
+    st.dec = V3{1.f, 1.f, 1.f};
+    st.col = V3{0.f, 0.f, 0.f};
+    st.dep = 0;
+}
+// One vertex of the path in `st` (closest hit, shading, shadow ray); false once the path has ended.
+__device__ __forceinline__ bool mega_bounce(const DScene &S, const DRender &R, RegPath &st) {
+    HitRec a{kINF, -1, 0.f, 0.f, 0};
+    analytic_tests<false>(S, st.o, st.d, AlphaCtx{st.key, st.dep, true}, a);
+    st.h = HitRec{a.t, -1, a.u, a.v, a.loc};
+    const ShadeOut r = shade_path(S, R, st, nullptr);
+    if (r.emit_shadow) {
+        HitRec b{r.s_tmax, -1, 0.f, 0.f, 0};
+        if (!analytic_tests<true>(S, r.s_org, r.s_dir, AlphaCtx{0, 0, false}, b))
+            st.col = V3{st.col.x + r.s_val.x, st.col.y + r.s_val.y, st.col.z + r.s_val.z};
+    }
+    return r.alive;
+}
+template <bool REFILL>
 __global__ TPT_SHADE_ATTR void k_megapath(DScene S, DRender R, DPaths P, int wave_base, int n_paths,
                                           uint64_t base_mixed) {
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_paths; j += gridDim.x * blockDim.x) {
+    const int stride = gridDim.x * blockDim.x;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (REFILL) {
+        if (j >= n_paths) return;
         RegPath st;
-        Rng rng;
-        camera_ray(S, R, j % R.pixels, wave_base + j / R.pixels, base_mixed, st.o, st.d, st.key, rng);
-        st.slot = j;
-        st.rs = rng.x;
-        st.dec = V3{1.f, 1.f, 1.f};
-        st.col = V3{0.f, 0.f, 0.f};
-        st.dep = 0;
+        mega_start(S, R, j, wave_base, base_mixed, st);
         while (true) {
-            HitRec a{kINF, -1, 0.f, 0.f, 0};
-            analytic_tests<false>(S, st.o, st.d, AlphaCtx{st.key, st.dep, true}, a);
-            st.h = HitRec{a.t, -1, a.u, a.v, a.loc};
-            const ShadeOut r = shade_path(S, R, st, nullptr);
-            if (r.emit_shadow) {
-                HitRec b{r.s_tmax, -1, 0.f, 0.f, 0};
-                if (!analytic_tests<true>(S, r.s_org, r.s_dir, AlphaCtx{0, 0, false}, b))
-                    st.col = V3{st.col.x + r.s_val.x, st.col.y + r.s_val.y, st.col.z + r.s_val.z};
+            if (!mega_bounce(S, R, st)) {
+                P.color[j] = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
+                j += stride;
+                if (j >= n_paths) break;
+                mega_start(S, R, j, wave_base, base_mixed, st);
             }
-            if (!r.alive) break;
         }
-        P.color[j] = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
+    } else {
+        for (; j < n_paths; j += stride) {
+            RegPath st;
+            mega_start(S, R, j, wave_base, base_mixed, st);
+            while (mega_bounce(S, R, st)) {
+            }
+            P.color[j] = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
+        }
     }
 }
 
@@ -1888,8 +1918,11 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
 }
 void launch_megapath(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
-                     uint64_t base_mixed, int grid, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_megapath, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+                     uint64_t base_mixed, int grid, int block, bool refill, hipStream_t st) {
+    if (refill)
+        hipLaunchKernelGGL(k_megapath<true>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+    else
+        hipLaunchKernelGGL(k_megapath<false>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
 }
 // Compile-time dispatch on the node layout and on boolean kernel variants.
 template <int V> using IntC = std::integral_constant<int, V>;
