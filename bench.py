@@ -62,9 +62,9 @@ EXTEND_COUNTERS = {"valu_insts": 1122033576, "thread_cycles": 37290147763, "acti
 CITY_EXTEND_COUNTERS = {"valu_insts": 1413430719, "thread_cycles": 49919942694, "active_inst": 1417850773,
                         "fetch_kib": 3085105.1, "write_kib": 262906.4}
 # Two-spheres box 512x384, 64 spp, depth 10, RR 5: the whole-path kernel's counters for one
-# render (one dispatch of all 12.6M paths; 99% of the render's kernel time).
-SPHERES_PATH_COUNTERS = {"valu_insts": 3189132576, "thread_cycles": 71627813901, "active_inst": 3282835014,
-                         "fetch_kib": 413.4, "write_kib": 196608.0, "dispatches": 1}
+# render (one dispatch of all 12.6M paths, lanes refilled; 99% of the render's kernel time).
+SPHERES_PATH_COUNTERS = {"valu_insts": 2447618600, "thread_cycles": 72325034179, "active_inst": 2517330930,
+                         "fetch_kib": 476.7, "write_kib": 436596.3, "dispatches": 1}
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 DEFAULT_ROOM = (1024, 768, 256)
 

@@ -25,7 +25,7 @@
 
 namespace tpt {
 void launch_generate(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, hipStream_t);
-void launch_megapath(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, int, bool, hipStream_t);
+void launch_megapath(const DScene &, const DRender &, const DPaths &, int, int, uint64_t, int, int, int, hipStream_t);
 void launch_extend(const DScene &, const DPaths &, const int32_t *, int, int, int, int, bool, uint64_t, hipStream_t);
 void launch_shade(const DScene &, const DRender &, const DPaths &, const int32_t *, int32_t *, int, int, int, int,
                   hipStream_t, const DCache *);
@@ -556,9 +556,13 @@ struct PlainWaves {
     int npools = 1, block = 256, cus = 256;
     bool count = false, timing = false;
     bool megakernel = env_int("TPT_MEGAKERNEL", 1, 0, 1) != 0;
-    // k_megapath with lane refill: waves per CU of its grid (0, the default and the
-    // fastest measured: one thread per path, no refill)
-    int mega_waves = env_int("TPT_MEGA_WAVES", 0, 0, 1024);
+    // k_megapath with lane refill: waves per CU of its grid (0: one thread per path,
+    // no refill) and the idle lanes of a wave that start their next paths together
+    // (two-spheres box 512x384x64: one thread per path 4.65 ms; 32 waves per CU
+    // refilling at 1 / 2 / 4 / 8 / 16 / 24 idle lanes 4.15 / 4.20 / 4.25-4.30 /
+    // 4.28 / 4.38 / 4.55 ms; 16, 24, 48 and 64 waves per CU no better)
+    int mega_waves = env_int("TPT_MEGA_WAVES", 32, 0, 1024);
+    int mega_refill = env_int("TPT_MEGA_REFILL_LANES", 1, 1, 64);
     std::vector<std::pair<int, int>> ext_ev, shade_ev, shadow_ev;
     int waves = 0;
     static int record(EventVec &evs, hipStream_t s, bool timing) {
@@ -584,7 +588,7 @@ struct PlainWaves {
                 int e0 = timing ? ev() : -1;
                 const bool refill = mega_waves > 0;
                 const int mgrid = refill ? std::max(1, std::min(grid, cus * mega_waves * 64 / block)) : grid;
-                launch_megapath(S, R, P, wave_base, n, base_mixed, mgrid, block, refill, sw);
+                launch_megapath(S, R, P, wave_base, n, base_mixed, mgrid, block, refill ? mega_refill : 0, sw);
                 if (timing) shade_ev.push_back({e0, ev()});
             } else {
                 launch_generate(S, R, P, wave_base, n, base_mixed, block, sw);

@@ -1264,13 +1264,13 @@ struct RegPath {
 // and the radiance is added in the wavefront's order (the vertex's emission and
 // BxDF sample in shade_path, then its shadow ray's contribution), so each path's
 // colour equals the wavefront kernels' bit for bit.  Writes it to P.color[j] for
-// k_accumulate.  REFILL: a lane whose path ends starts its next path (j + the
-// grid's thread count) at the next bounce, instead of waiting for the wave's
-// longest path; the grid is then sized to the resident waves.  Measured slower
-// on the two-spheres box 512x384x64 (depth 10, RR 5; same images): one thread
-// per path 4.66 ms per render, refill with 8 / 16 / 32 / 64 waves per CU 8.3 /
-// 5.5 / 5.2 / 4.95 ms.  New waves starting where others end keep the SIMDs as
-// busy as refilled lanes do, without the divergent path starts.
+// k_accumulate.  REFILL: a lane whose path has ended starts its next path (j +
+// the grid's thread count) instead of idling until the wave's longest path ends;
+// the idle lanes start at the top of the loop, once refill_lanes of them are idle
+// (a wave-uniform decision).  Two-spheres box 512x384x64 (depth 10, RR 5; same
+// images): one thread per path 4.65 ms per render, refill at 1 / 4 / 16 idle
+// lanes 4.15 / 4.25-4.30 / 4.38 ms; a lane restarting inside the bounce's
+// path-ended branch instead took 5.0-8.3 ms.
 __device__ __forceinline__ void mega_start(const DScene &S, const DRender &R, int j, int wave_base,
                                            uint64_t base_mixed, RegPath &st) {
     Rng rng;
@@ -1296,19 +1296,28 @@ __device__ __forceinline__ bool mega_bounce(const DScene &S, const DRender &R, R
 }
 template <bool REFILL>
 __global__ TPT_SHADE_ATTR void k_megapath(DScene S, DRender R, DPaths P, int wave_base, int n_paths,
-                                          uint64_t base_mixed) {
+                                          uint64_t base_mixed, int refill_lanes) {
     const int stride = gridDim.x * blockDim.x;
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (REFILL) {
-        if (j >= n_paths) return;
+        // idle lanes start their next paths together, once at least refill_lanes
+        // of the wave's lanes are idle or none is tracing
         RegPath st;
-        mega_start(S, R, j, wave_base, base_mixed, st);
+        bool live = false;
         while (true) {
-            if (!mega_bounce(S, R, st)) {
+            const uint64_t idle = __ballot(!live && j < n_paths);
+            const uint64_t act = __ballot(live);
+            if (idle == 0 && act == 0) break;
+            if (__popcll(idle) >= refill_lanes || act == 0) {
+                if (!live && j < n_paths) {
+                    mega_start(S, R, j, wave_base, base_mixed, st);
+                    live = true;
+                }
+            }
+            if (live && !mega_bounce(S, R, st)) {
                 P.color[j] = make_float4(st.col.x, st.col.y, st.col.z, 0.f);
+                live = false;
                 j += stride;
-                if (j >= n_paths) break;
-                mega_start(S, R, j, wave_base, base_mixed, st);
             }
         }
     } else {
@@ -1918,11 +1927,13 @@ void launch_generate(const DScene &S, const DRender &R, const DPaths &P, int wav
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
 }
 void launch_megapath(const DScene &S, const DRender &R, const DPaths &P, int wave_base, int n_paths,
-                     uint64_t base_mixed, int grid, int block, bool refill, hipStream_t st) {
-    if (refill)
-        hipLaunchKernelGGL(k_megapath<true>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+                     uint64_t base_mixed, int grid, int block, int refill_lanes, hipStream_t st) {
+    if (refill_lanes > 0)
+        hipLaunchKernelGGL(k_megapath<true>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed,
+                           refill_lanes);
     else
-        hipLaunchKernelGGL(k_megapath<false>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed);
+        hipLaunchKernelGGL(k_megapath<false>, dim3(grid), dim3(block), 0, st, S, R, P, wave_base, n_paths, base_mixed,
+                           0);
 }
 // Compile-time dispatch on the node layout and on boolean kernel variants.
 template <int V> using IntC = std::integral_constant<int, V>;
